@@ -1,0 +1,376 @@
+"""hcorepp_amd -- MI355X-native TLR tile GEMM + recompression (HCore++ hot path).
+
+Python host-side mirror of the reference's operator interface, over the C-ABI
+of ``include/hcorepp_amd.h`` (library ``hcorepp_amd/libhcorepp_amd.so``):
+
+    reference (C++)                               here
+    hcorepp::api::HCore<T>::Gemm                  HCore.gemm / gemm()
+    hcorepp::operators::DenseTile<T>              DenseTile
+    hcorepp::operators::CompressedTile<T>         CompressedTile
+    hcorepp::operators::CompressionParameters     CompressionParameters
+    hcorepp::kernels::RunContext (+ MemoryUnit)   RunContext (stream + device arena)
+
+Device buffers are allocated through the library itself (hcorepp_malloc /
+hcorepp_memcpy), so one HIP runtime serves the whole process.  The library must
+be built (``hcorepp_amd.build.build()``); every compute entry point raises if it
+is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libhcorepp_amd.so")
+
+DENSE, COMPRESSED = 0, 1
+GESVD, GESDD = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+STATUS = {0: "OK", -1: "INVALID", -2: "UNSUPPORTED", -3: "CAPACITY", -4: "HIP", -5: "NUMERIC", -6: "NOMEM"}
+
+
+class HCoreError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _Tile(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("rank", C.c_int64), ("max_rank", C.c_int64),
+                ("ld", C.c_int64), ("layout", C.c_int32), ("type", C.c_int32), ("data", C.c_void_p)]
+
+
+class _CParams(C.Structure):
+    _fields_ = [("accuracy", C.c_double), ("trmm", C.c_int32), ("ungqr", C.c_int32),
+                ("truncated_svd", C.c_int32), ("svd_type", C.c_int32), ("fixed_rank", C.c_int64)]
+
+
+# Every symbol include/hcorepp_amd.h declares.
+EXPORTED = [
+    "hcorepp_ctx_create", "hcorepp_ctx_destroy", "hcorepp_ctx_sync", "hcorepp_ctx_stream",
+    "hcorepp_ctx_arena_bytes", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy", "hcorepp_memset",
+    "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
+    "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
+    "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
+    "hcorepp_batch_workspace_bytes", "hcorepp_batch_destroy", "hcorepp_last_error", "hcorepp_version",
+]
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with hcorepp_amd.build.build()")
+    L = C.CDLL(LIB_PATH)
+    P, I64, I32, D = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+    L.hcorepp_ctx_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(P)]
+    L.hcorepp_ctx_destroy.argtypes = [P]
+    L.hcorepp_ctx_sync.argtypes = [P]
+    L.hcorepp_ctx_stream.argtypes = [P]
+    L.hcorepp_ctx_stream.restype = P
+    L.hcorepp_ctx_arena_bytes.argtypes = [P]
+    L.hcorepp_ctx_arena_bytes.restype = C.c_size_t
+    L.hcorepp_memcpy.argtypes = [P, P, P, C.c_size_t]
+    L.hcorepp_memset.argtypes = [P, P, C.c_int, C.c_size_t]
+    L.hcorepp_malloc.argtypes = [C.c_size_t, C.POINTER(P)]
+    L.hcorepp_free.argtypes = [P]
+    L.hcorepp_ctx_event_record.argtypes = [P, C.c_int]
+    L.hcorepp_ctx_event_elapsed.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.hcorepp_gemm_d.argtypes = [P, D, C.POINTER(_Tile), C.c_char, C.POINTER(_Tile), C.c_char, D,
+                                 C.POINTER(_Tile), C.POINTER(_CParams), C.c_int, C.POINTER(C.c_uint64), P,
+                                 C.POINTER(I64)]
+    L.hcorepp_gemm_s.argtypes = [P, C.c_float, C.POINTER(_Tile), C.c_char, C.POINTER(_Tile), C.c_char,
+                                 C.c_float, C.POINTER(_Tile), C.POINTER(_CParams), C.c_int,
+                                 C.POINTER(C.c_uint64), P, C.POINTER(I64)]
+    L.hcorepp_gemm_pool_size_d.argtypes = [C.POINTER(_Tile)] * 3 + [C.POINTER(_CParams)]
+    L.hcorepp_gemm_pool_size_d.restype = C.c_size_t
+    L.hcorepp_batch_create.argtypes = [P, I64, C.POINTER(_Tile), C.POINTER(_Tile), C.POINTER(_Tile), D, D,
+                                       C.POINTER(_CParams), C.c_int, C.POINTER(P)]
+    for name in ("hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
+                 "hcorepp_batch_destroy"):
+        getattr(L, name).argtypes = [P]
+    L.hcorepp_batch_rank_dev.argtypes = [P]
+    L.hcorepp_batch_rank_dev.restype = P
+    L.hcorepp_batch_ranks.argtypes = [P, P]
+    L.hcorepp_batch_set_ranks.argtypes = [P, P]
+    L.hcorepp_batch_status.argtypes = [P, C.POINTER(I64)]
+    L.hcorepp_batch_workspace_bytes.argtypes = [P]
+    L.hcorepp_batch_workspace_bytes.restype = C.c_size_t
+    L.hcorepp_last_error.restype = C.c_char_p
+    L.hcorepp_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(st: int):
+    if st != 0:
+        raise HCoreError(st, lib().hcorepp_last_error().decode())
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (hcorepp_malloc / hcorepp_free)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        _check(lib().hcorepp_malloc(max(self.nbytes, 1), C.byref(h)))
+        self.ptr = h.value
+
+    @staticmethod
+    def from_numpy(a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset_bytes: int = 0):
+        a = np.ascontiguousarray(a)
+        assert offset_bytes + a.nbytes <= self.nbytes
+        _check(lib().hcorepp_memcpy(None, self.ptr + offset_bytes, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def download(self, dtype, count: int | None = None, offset_bytes: int = 0) -> np.ndarray:
+        itemsize = np.dtype(dtype).itemsize
+        if count is None:
+            count = (self.nbytes - offset_bytes) // itemsize
+        out = np.empty(count, dtype=dtype)
+        if count:
+            _check(lib().hcorepp_memcpy(None, out.ctypes.data_as(C.c_void_p), self.ptr + offset_bytes,
+                                        out.nbytes))
+        return out
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None):
+        n = self.nbytes if nbytes is None else nbytes
+        _check(lib().hcorepp_memcpy(None, self.ptr, other.ptr, n))
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().hcorepp_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CompressionParameters:
+    """Mirror of hcorepp::operators::CompressionParameters
+    (include/hcorepp/operators/helpers/CompressionParameters.hpp:44-46 defaults)."""
+
+    def __init__(self, accuracy=1e-4, trmm=False, ungqr=True, truncated_svd=False, fixed_rank=0,
+                 svd_type=GESDD):
+        self.accuracy = float(accuracy)
+        self.trmm, self.ungqr, self.truncated_svd = bool(trmm), bool(ungqr), bool(truncated_svd)
+        self.fixed_rank, self.svd_type = int(fixed_rank), int(svd_type)
+
+    def c(self) -> _CParams:
+        return _CParams(self.accuracy, int(self.trmm), int(self.ungqr), int(self.truncated_svd),
+                        self.svd_type, self.fixed_rank)
+
+
+class RunContext:
+    """One HIP stream + device arena (RunContext + pool MemoryUnit of the reference)."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = 0):
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().hcorepp_ctx_create(device, arena_bytes, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        _check(lib().hcorepp_ctx_sync(self.h))
+
+    @property
+    def stream_ptr(self) -> int:
+        return lib().hcorepp_ctx_stream(self.h) or 0
+
+    def arena_bytes(self) -> int:
+        return lib().hcorepp_ctx_arena_bytes(self.h)
+
+    def record(self, slot: int):
+        _check(lib().hcorepp_ctx_event_record(self.h, slot))
+
+    def elapsed_ms(self, slot0: int, slot1: int) -> float:
+        ms = C.c_float(0)
+        _check(lib().hcorepp_ctx_event_elapsed(self.h, slot0, slot1, C.byref(ms)))
+        return float(ms.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hcorepp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DTYPES = {"d": np.float64, "s": np.float32}
+
+
+class _TileBase:
+    dtype = np.float64
+
+    def c(self) -> _Tile:
+        return _Tile(self.rows, self.cols, self.rank, self.max_rank, self.ld, 0, self.type, self.buf.data_ptr())
+
+
+class DenseTile(_TileBase):
+    """Dense tile in HBM (DenseTile<T>, reference src/operators/concrete/Dense.cpp:19-29)."""
+    type = DENSE
+
+    def __init__(self, a, device=0, dtype=np.float64):
+        a = np.asfortranarray(np.asarray(a, dtype=dtype))
+        self.rows, self.cols = a.shape
+        self.ld = max(self.rows, 1)
+        self.rank = self.max_rank = min(self.rows, self.cols)
+        self.dtype = dtype
+        self.buf = DeviceBuffer.from_numpy(a.reshape(-1, order="F"))
+
+    def to_numpy(self) -> np.ndarray:
+        h = self.buf.download(self.dtype)
+        return h[: self.ld * self.cols].reshape(self.ld, self.cols, order="F")[: self.rows].copy()
+
+    to_dense = to_numpy
+
+
+class CompressedTile(_TileBase):
+    """Compressed tile U*V in HBM with the reference layout
+    [U rows x max_rank | V max_rank x cols] (src/operators/concrete/Compressed.cpp:19-47,175-185)."""
+    type = COMPRESSED
+
+    def __init__(self, u, v, max_rank=None, device=0, dtype=np.float64):
+        u = np.asarray(u, dtype=dtype)
+        v = np.asarray(v, dtype=dtype)
+        self.rows, r = u.shape
+        r2, self.cols = v.shape
+        assert r == r2, "U and V ranks differ"
+        self.rank = r
+        self.max_rank = r if max_rank is None else int(max_rank)
+        assert self.max_rank >= r
+        self.ld = self.rows
+        self.dtype = dtype
+        host = np.zeros((self.rows + self.cols) * max(self.max_rank, 1), dtype=dtype)
+        host[: self.rows * r] = u.reshape(-1, order="F")
+        off = self.rows * self.max_rank
+        host[off: off + r * self.cols] = v.reshape(-1, order="F")
+        self.buf = DeviceBuffer.from_numpy(host)
+
+    def factors(self):
+        h = self.buf.download(self.dtype)
+        u = h[: self.rows * self.rank].reshape(self.rows, self.rank, order="F").copy()
+        off = self.rows * self.max_rank
+        v = h[off: off + self.rank * self.cols].reshape(self.rank, self.cols, order="F").copy()
+        return u, v
+
+    def to_dense(self) -> np.ndarray:
+        u, v = self.factors()
+        return u.astype(np.float64) @ v.astype(np.float64)
+
+    def update_from(self, t: _Tile):
+        self.rows, self.cols, self.rank, self.max_rank, self.ld = t.rows, t.cols, t.rank, t.max_rank, t.ld
+
+
+class HCore:
+    """hcorepp::api::HCore<T> (reference include/hcorepp/api/HCore.hpp:23-128)."""
+
+    @staticmethod
+    def gemm(alpha, A, B, beta, Ct, ctx: RunContext, params: CompressionParameters | None = None,
+             opA="N", opB="N", cholesky=False, want_sigma=False):
+        """C = alpha*op(A)*op(B) + beta*C (reference src/api/HCore.cpp:22-344).  Returns the
+        reference flop count, and the sorted core singular values if ``want_sigma``."""
+        L = lib()
+        p = (params or CompressionParameters(1e-9)).c()
+        ta, tb, tc = A.c(), B.c(), Ct.c()
+        flops = C.c_uint64(0)
+        nsig = C.c_int64(0)
+        sig = np.zeros(max(Ct.rows, Ct.cols, 1) + 1024) if want_sigma else None
+        sp = sig.ctypes.data_as(C.c_void_p) if want_sigma else None
+        f = L.hcorepp_gemm_s if Ct.dtype == np.float32 else L.hcorepp_gemm_d
+        st = f(ctx.h, alpha, C.byref(ta), opA.encode(), C.byref(tb), opB.encode(), beta, C.byref(tc),
+               C.byref(p), int(bool(cholesky)), C.byref(flops), sp, C.byref(nsig))
+        _check(st)
+        if isinstance(Ct, CompressedTile):
+            Ct.update_from(tc)
+        if want_sigma:
+            return flops.value, sig[: nsig.value].copy()
+        return flops.value
+
+
+gemm = HCore.gemm
+
+
+class Batch:
+    """Batched variable-rank form: n independent updates C_i = alpha*A_i*B_i + beta*C_i
+    run by one persistent launch (optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, ctx: RunContext, As, Bs, Cs, alpha=1.0, beta=1.0,
+                 params: CompressionParameters | None = None, precision=PREC_FP64):
+        L = lib()
+        n = len(Cs)
+        self.ctx, self.n = ctx, n
+        self._tiles = (As, Bs, Cs)  # keep device buffers alive
+        TA = (_Tile * n)(*[a.c() for a in As])
+        TB = (_Tile * n)(*[b.c() for b in Bs])
+        TC = (_Tile * n)(*[c.c() for c in Cs])
+        p = (params or CompressionParameters(1e-9)).c()
+        h = C.c_void_p()
+        _check(L.hcorepp_batch_create(ctx.h, n, TA, TB, TC, float(alpha), float(beta), C.byref(p),
+                                      int(precision), C.byref(h)))
+        self.h = h
+
+    def run(self):
+        _check(lib().hcorepp_batch_run(self.h))
+
+    def capture(self):
+        _check(lib().hcorepp_batch_capture_graph(self.h))
+
+    def launch_graph(self):
+        _check(lib().hcorepp_batch_launch_graph(self.h))
+
+    def ranks(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.int64)
+        _check(lib().hcorepp_batch_ranks(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_ranks(self, ranks):
+        r = np.ascontiguousarray(ranks, dtype=np.int64)
+        _check(lib().hcorepp_batch_set_ranks(self.h, r.ctypes.data_as(C.c_void_p)))
+
+    def status(self):
+        sw = C.c_int64(0)
+        st = lib().hcorepp_batch_status(self.h, C.byref(sw))
+        return st, sw.value
+
+    def workspace_bytes(self) -> int:
+        return lib().hcorepp_batch_workspace_bytes(self.h)
+
+    def sync_ranks_to_tiles(self):
+        rk = self.ranks()
+        for c, r in zip(self._tiles[2], rk):
+            c.rank = int(r)
+        return rk
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hcorepp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

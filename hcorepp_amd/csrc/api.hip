@@ -1,0 +1,770 @@
+// Host side of the C-ABI (include/hcorepp_amd.h): execution contexts with a
+// device arena, the plain one-update path (HCore<T>::Gemm restated for the
+// device), and the batched variable-rank form.
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hcorepp_amd.h"
+#include "kernels.hpp"
+
+using namespace hcamd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+hcorepp_status fail(hcorepp_status st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+hcorepp_status fail(hcorepp_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(HCOREPP_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Dynamic LDS per workgroup for the QR / SVD / reconstruction / batched kernels:
+// 80 KB lets two workgroups share one CU's 160 KB.
+constexpr int64_t kLdsBytes = 80 * 1024;
+
+}  // namespace
+
+struct hcorepp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char *arena = nullptr;
+    size_t arena_cap = 0;
+    size_t arena_off = 0;
+    int64_t *pinned = nullptr;  // host mirror for rank/status readback (HDR_WORDS int64)
+    double *pinned_sigma = nullptr;
+    size_t pinned_sigma_cap = 0;
+    int num_cus = 0;
+    hipEvent_t events[16] = {};
+};
+
+namespace {
+
+bool g_lds_ready_d = false, g_lds_ready_s = false;
+std::mutex g_lds_mutex;
+
+hcorepp_status ensure_lds_limits() {
+    std::lock_guard<std::mutex> lk(g_lds_mutex);
+    if (!g_lds_ready_d) {
+        HIP_TRY(set_kernel_lds_limits<double>(kLdsBytes));
+        g_lds_ready_d = true;
+    }
+    if (!g_lds_ready_s) {
+        HIP_TRY(set_kernel_lds_limits<float>(kLdsBytes));
+        g_lds_ready_s = true;
+    }
+    return HCOREPP_OK;
+}
+
+// Reserve `bytes` of arena for the next operation: grows (after draining the
+// stream) when the request exceeds the current capacity, then resets the bump pointer.
+hcorepp_status arena_begin(hcorepp_ctx *ctx, size_t bytes) {
+    ctx->arena_off = 0;
+    if (bytes <= ctx->arena_cap) return HCOREPP_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->arena) HIP_TRY(hipFree(ctx->arena));
+    ctx->arena = nullptr;
+    size_t cap = std::max(bytes, ctx->arena_cap * 2);
+    if (hipMalloc(&ctx->arena, cap) != hipSuccess) {
+        ctx->arena = nullptr;
+        ctx->arena_cap = 0;
+        return fail(HCOREPP_ERR_NOMEM, "arena allocation of %zu bytes failed", cap);
+    }
+    ctx->arena_cap = cap;
+    return HCOREPP_OK;
+}
+
+void *arena_take(hcorepp_ctx *ctx, size_t bytes) {
+    size_t off = (ctx->arena_off + 255) & ~size_t(255);
+    ctx->arena_off = off + bytes;
+    return ctx->arena + off;
+}
+
+size_t aligned(size_t b) { return (b + 255) & ~size_t(255); }
+
+// (rows, cols, ld, ptr) view of a tile factor, as DataHolder (reference src/data-units/DataHolder.cpp:12-30).
+template <typename T>
+struct Holder {
+    int64_t rows, cols, ld;
+    T *data;
+};
+
+template <typename T>
+int tile_holders(const hcorepp_tile *t, char op, Holder<T> *h, char *hop) {
+    if (t->type == HCOREPP_DENSE) {
+        h[0] = {t->rows, t->cols, t->ld, (T *) t->data};
+        hop[0] = op;
+        return 1;
+    }
+    Holder<T> u = {t->rows, t->rank, t->rows, (T *) t->data};
+    Holder<T> v = {t->rank, t->cols, t->rank, (T *) t->data + t->rows * t->max_rank};
+    if (op == 'T') {
+        h[0] = v;
+        h[1] = u;
+    } else {
+        h[0] = u;
+        h[1] = v;
+    }
+    hop[0] = hop[1] = op;
+    return 2;
+}
+
+// DenseTile<T>::Gemm k / ld inference (reference src/operators/concrete/Dense.cpp:55-96, ColMajor).
+template <typename T>
+hipError_t dense_tile_gemm(hipStream_t s, int64_t m, int64_t n, T *c, int64_t ldc, T alpha, const Holder<T> &a,
+                           char opa, const Holder<T> &b, char opb, T beta) {
+    int64_t k = std::min(a.cols, b.rows);
+    if (opa == 'T') {
+        k = std::min(a.rows, b.rows);
+        if (opb == 'T') k = std::min(a.rows, b.cols);
+    } else if (opb == 'T') {
+        k = std::min(a.cols, b.cols);
+    }
+    const int64_t lda = (opa == 'T') ? std::max(a.cols, k) : std::max(a.rows, m);
+    const int64_t ldb = (opb == 'T') ? std::max(b.cols, n) : std::max(b.rows, k);
+    return launch_gemm<T>(s, (int) m, (int) n, (int) k, alpha, a.data, lda, opa == 'T', b.data, ldb, opb == 'T',
+                          beta, c, ldc);
+}
+
+// Flop model of CompressedTile<T>::Gemm, default branch (reference Compressed.cpp:335-625).
+uint64_t ref_flops_compressed(uint64_t m, uint64_t n, uint64_t arank, uint64_t crk, uint64_t rk, bool fixed) {
+    const uint64_t Um = m, Un = arank + crk, minU = std::min(Um, Un);
+    const uint64_t Vm = n, Vn = arank + crk, minV = std::min(Vm, Vn);
+    const uint64_t sizeS = std::min(std::min(m, n), arank + crk);
+    uint64_t f = 0;
+    f += m * crk + m * arank + m * arank;
+    f += 2 * Um * Un * Un - (uint64_t) (2.0f * (float) Un * (float) Un * (float) Un / 3.0f) + 2 * Um * Un +
+         (uint64_t) (17 * (float) Un / 3.0f);
+    f += 2 * (minU * ((Un + 1) / 2));
+    f += n * crk + n * arank;
+    f += 2 * Vm * Vn * Vn - (uint64_t) (2 * (float) Vn * (float) Vn * (float) Vn / 3.0f) + 2 * Vm * Vn +
+         (uint64_t) (17 * (float) Vn / 3.0f);
+    f += 2 * (minV * ((Vn + 1) / 2));
+    f += minU * minV * (arank + crk);
+    f += 22 * minV * minV * minV;
+    if (!fixed) f += sizeS;
+    f += 2 * (uint64_t) std::pow((double) minU, 2) * (3 * Um - minU);  // (8/3) evaluates to 2
+    f += Um * rk * minU;
+    f += rk;
+    f += 2 * (uint64_t) std::pow((double) minV, 2) * (3 * Vm - minV);
+    f += Vm * rk * minV;
+    return f;
+}
+
+template <typename T>
+hcorepp_status validate_tile(const hcorepp_tile *t, const char *name) {
+    if (!t) return fail(HCOREPP_ERR_INVALID, "%s is null", name);
+    if (t->layout != 0) return fail(HCOREPP_ERR_UNSUPPORTED, "%s: only ColMajor layout is implemented", name);
+    if (t->rows < 0 || t->cols < 0) return fail(HCOREPP_ERR_INVALID, "%s: negative size", name);
+    if (t->type == HCOREPP_DENSE) {
+        if (t->ld < std::max<int64_t>(t->rows, 1)) return fail(HCOREPP_ERR_INVALID, "%s: ld < rows", name);
+    } else if (t->type == HCOREPP_COMPRESSED) {
+        if (t->rank < 0 || t->max_rank < t->rank)
+            return fail(HCOREPP_ERR_INVALID, "%s: rank %lld outside [0, max_rank %lld]", name, (long long) t->rank,
+                        (long long) t->max_rank);
+    } else {
+        return fail(HCOREPP_ERR_INVALID, "%s: unknown tile type %d", name, t->type);
+    }
+    if (!t->data && t->rows * t->cols > 0) return fail(HCOREPP_ERR_INVALID, "%s: null data", name);
+    return HCOREPP_OK;
+}
+
+template <typename T>
+hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char opA, const hcorepp_tile *B,
+                          char opB, T beta, hcorepp_tile *C, const hcorepp_cparams *params, int cholesky,
+                          uint64_t *flops, double *sigma_out, int64_t *n_sigma) {
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    hcorepp_status st;
+    if ((st = validate_tile<T>(A, "A")) || (st = validate_tile<T>(B, "B")) || (st = validate_tile<T>(C, "C")))
+        return st;
+    if ((opA != 'N' && opA != 'T') || (opB != 'N' && opB != 'T'))
+        return fail(HCOREPP_ERR_INVALID, "op must be 'N' or 'T'");
+    hcorepp_cparams p = params ? *params : hcorepp_cparams{1e-9, 0, 1, 0, HCOREPP_SVD_GESDD, 0};
+    if ((st = ensure_lds_limits())) return st;
+    if (n_sigma) *n_sigma = 0;
+    hipStream_t s = ctx->stream;
+
+    Holder<T> ah[2], bh[2];
+    char aop[2], bop[2];
+    const int na = tile_holders<T>(A, opA, ah, aop);
+    const int nb = tile_holders<T>(B, opB, bh, bop);
+    const int nc = (C->type == HCOREPP_DENSE) ? 1 : 2;
+    const int total = na + nb;
+    const bool ddc = (total == 2 && nc == 2);
+    int ntemp = (total == 3 || ddc) ? 1 : (total == 4 ? 2 : 0);
+
+    if (cholesky)
+        return fail(HCOREPP_ERR_UNSUPPORTED,
+                    "aCholesky operand convention (reference HCore.cpp:160-174) is not in this build");
+    if (C->type == HCOREPP_COMPRESSED && !ddc && (opA != 'N' || opB != 'N'))
+        return fail(HCOREPP_ERR_UNSUPPORTED,
+                    "compressed C with transposed operands: the reference CompressedTile::Gemm ignores the ops");
+
+    // ---- size the arena for this update ----
+    size_t need = 0;
+    {
+        int ai = na - 1, bi = 0, nt = ntemp;
+        Holder<T> a2[2] = {ah[0], ah[1]}, b2[2] = {bh[0], bh[1]};
+        char ao2[2] = {aop[0], aop[1]}, bo2[2] = {bop[0], bop[1]};
+        while (nt > 0) {
+            const int64_t mm = (ao2[ai] == 'T') ? a2[ai].cols : a2[ai].rows;
+            const int64_t nn = (bo2[bi] == 'T') ? b2[bi].rows : b2[bi].cols;
+            need += aligned(sizeof(T) * std::max<int64_t>(mm * nn, 1));
+            Holder<T> th = {mm, nn, mm, nullptr};
+            if (nt == 2 || (nt == 1 && ai == 0)) {
+                a2[ai] = th;
+                ao2[ai] = 'N';
+                bi++;
+            } else {
+                b2[bi] = th;
+                bo2[bi] = 'N';
+                ai--;
+            }
+            nt--;
+        }
+        if (C->type == HCOREPP_COMPRESSED && !ddc) {
+            const int64_t arank = a2[ai].cols;
+            const int64_t kmax = arank + C->rank;
+            const WsLayout L = ws_layout<T>(C->rows, C->cols, kmax, C->max_rank);
+            need += aligned(sizeof(T) * L.total) + aligned(sizeof(int64_t) * 8) +
+                    aligned(sizeof(double) * (size_t) std::max<int64_t>(kmax, 1));
+        }
+    }
+    if ((st = arena_begin(ctx, need + 4096))) return st;
+
+    // ---- temporary factor products (HCore.cpp:176-264) ----
+    int a_idx = na - 1, b_idx = 0;
+    T *temps[2] = {nullptr, nullptr};
+    int64_t tm[2] = {0, 0}, tn[2] = {0, 0};
+    int ntt = 0;
+    const T alpha_local = ddc ? alpha : T(1), beta_local = T(0);
+    while (ntemp > 0) {
+        Holder<T> ad = ah[a_idx], bd = bh[b_idx];
+        const char ao = aop[a_idx], bo = bop[b_idx];
+        const int64_t mm = (ao == 'T') ? ad.cols : ad.rows;
+        const int64_t nn = (bo == 'T') ? bd.rows : bd.cols;
+        T *t = (T *) arena_take(ctx, sizeof(T) * std::max<int64_t>(mm * nn, 1));
+        temps[ntt] = t;
+        tm[ntt] = mm;
+        tn[ntt] = nn;
+        ntt++;
+        HIP_TRY(dense_tile_gemm<T>(s, mm, nn, t, mm, alpha_local, ad, ao, bd, bo, beta_local));
+        if (flops) *flops += 2ull * mm * nn * ad.cols;
+        Holder<T> th = {mm, nn, mm, t};
+        if (ntemp == 2 || (ntemp == 1 && a_idx == 0)) {
+            ah[a_idx] = th;
+            aop[a_idx] = 'N';
+            b_idx++;
+        } else {
+            bh[b_idx] = th;
+            bop[b_idx] = 'N';
+            a_idx--;
+        }
+        ntemp--;
+    }
+
+    if (ddc) {
+        // HCore.cpp:272-299: W += beta * C_U * C_V, then ReadjustTile(W, rank = -1).
+        T *W = temps[0];
+        const int64_t m = tm[0], n = tn[0];
+        Holder<T> cu = {C->rows, C->rank, C->rows, (T *) C->data};
+        Holder<T> cv = {C->rank, C->cols, C->rank, (T *) C->data + C->rows * C->max_rank};
+        HIP_TRY(dense_tile_gemm<T>(s, m, n, W, m, beta, cu, 'N', cv, 'N', T(1)));
+        if (flops) *flops += 2ull * m * n * cu.cols;
+        if (m < n)
+            return fail(HCOREPP_ERR_UNSUPPORTED,
+                        "DDC with rows < cols: the reference's identity fill writes past its V buffer");
+        if ((C->rows + C->cols) * C->max_rank < (m + n) * n)
+            return fail(HCOREPP_ERR_CAPACITY, "DDC result needs max_rank >= cols (%lld)", (long long) n);
+        HIP_TRY(launch_readjust_ddc<T>(s, W, m, n, (T *) C->data));
+        C->rows = m;
+        C->cols = n;
+        C->rank = n;
+        C->max_rank = n;
+        C->ld = m;
+        return HCOREPP_OK;
+    }
+
+    const Holder<T> ad = ah[a_idx], bd = bh[b_idx];
+    if (C->type == HCOREPP_DENSE) {
+        HIP_TRY(dense_tile_gemm<T>(s, C->rows, C->cols, (T *) C->data, C->ld, alpha, ad, aop[a_idx], bd,
+                                   bop[b_idx], beta));
+        if (flops) *flops += 2ull * C->rows * C->cols * ad.cols;
+        return HCOREPP_OK;
+    }
+    // ---- compressed C: recompression (CompressedTile<T>::Gemm) ----
+    const int64_t m = C->rows, n = C->cols, arank = ad.cols, rc = C->rank;
+    const int64_t K = arank + rc;
+    if (ad.rows != m || bd.cols != n || bd.rows != arank)
+        return fail(HCOREPP_ERR_INVALID, "operand shapes do not match C (%lldx%lld)", (long long) m,
+                    (long long) n);
+    const int pb = pick_pb(std::max(m, n), kLdsBytes / (int64_t) sizeof(T));
+    if (pb == 0) return fail(HCOREPP_ERR_UNSUPPORTED, "tile too tall for the LDS panel (rows %lld)", (long long) m);
+    const WsLayout L = ws_layout<T>(m, n, K, C->max_rank);
+    RecompDesc<T> d{};
+    d.afin = ad.data;
+    d.ldaf = ad.ld;
+    d.bfin = bd.data;
+    d.ldbf = arank;  // ProcessVpointer reads B with ld = arank (omp/kernels.cpp:69-76)
+    d.cdata = (T *) C->data;
+    d.crank = (int64_t *) arena_take(ctx, sizeof(int64_t) * 8);
+    d.ws = (T *) arena_take(ctx, sizeof(T) * L.total);
+    d.m = (int32_t) m;
+    d.n = (int32_t) n;
+    d.arank = (int32_t) arank;
+    d.max_rank = (int32_t) C->max_rank;
+    d.kmax = (int32_t) K;
+    d.truncated_svd = p.truncated_svd;
+    d.fixed_rank = (int32_t) p.fixed_rank;
+    d.alpha = alpha;
+    d.beta = beta;
+    d.accuracy = p.accuracy;
+    d.sigma_out = (double *) arena_take(ctx, sizeof(double) * (size_t) std::max<int64_t>(K, 1));
+    HIP_TRY(launch_set_i64(s, d.crank, rc));
+    HIP_TRY(launch_recomp_pre<T>(s, d, K, kLdsBytes, pb));
+    // The plain form reads the new rank back with one stream synchronisation,
+    // as the reference does (src/kernels/cuda/CudaKernels.cu:656-697).
+    const int64_t *hdr = reinterpret_cast<const int64_t *>(d.ws + L.hdr);
+    HIP_TRY(hipMemcpyAsync(ctx->pinned, hdr, sizeof(int64_t) * HDR_WORDS, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const int64_t r = ctx->pinned[HDR_RANK];
+    const int64_t size_s = ctx->pinned[HDR_SIZES];
+    const int64_t status = ctx->pinned[HDR_STATUS];
+    if (status & STATUS_NAN) return fail(HCOREPP_ERR_NUMERIC, "NaN/Inf in the core SVD");
+    if (status & STATUS_NOCONV) return fail(HCOREPP_ERR_NUMERIC, "core Jacobi SVD did not converge");
+    if (r < 1 || r > C->max_rank) return fail(HCOREPP_ERR_NUMERIC, "invalid rank %lld", (long long) r);
+    HIP_TRY(launch_recomp_post<T>(s, d, r, kLdsBytes, pb));
+    if (sigma_out && size_s > 0) {
+        HIP_TRY(hipMemcpyAsync(sigma_out, d.sigma_out, sizeof(double) * size_s, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (n_sigma) *n_sigma = size_s;
+    }
+    if (flops) *flops += ref_flops_compressed(m, n, arank, rc, r, p.fixed_rank != 0);
+    C->rank = r;
+    C->ld = m;
+    return HCOREPP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Batched variable-rank form
+// ===========================================================================
+struct hcorepp_batch {
+    hcorepp_ctx *ctx = nullptr;
+    int precision = 0;
+    int64_t n = 0;
+    int pb = 16;
+    int grid = 1;
+    void *entries = nullptr;  // BatchEntry<double>[n]
+    int32_t *order = nullptr;
+    int64_t *ranks = nullptr;
+    BatchCounters *ctr = nullptr;
+    char *ws = nullptr;
+    size_t ws_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+extern "C" {
+
+hcorepp_status hcorepp_ctx_create(int device, size_t arena_bytes, hcorepp_ctx **out) {
+    if (!out) return fail(HCOREPP_ERR_INVALID, "out is null");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(device));
+    hcorepp_ctx *c = new (std::nothrow) hcorepp_ctx();
+    if (!c) return fail(HCOREPP_ERR_NOMEM, "host allocation failed");
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);  // ordered with the null stream
+    if (e != hipSuccess) {
+        delete c;
+        return fail(HCOREPP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    e = hipHostMalloc((void **) &c->pinned, sizeof(int64_t) * 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(HCOREPP_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (arena_bytes) {
+        hcorepp_status st = arena_begin(c, arena_bytes);
+        if (st) {
+            hipHostFree(c->pinned);
+            hipStreamDestroy(c->stream);
+            delete c;
+            return st;
+        }
+    }
+    hcorepp_status st = ensure_lds_limits();
+    if (st) {
+        hipHostFree(c->pinned);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return st;
+    }
+    *out = c;
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_ctx_destroy(hcorepp_ctx *ctx) {
+    if (!ctx) return HCOREPP_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &ev : ctx->events)
+        if (ev) hipEventDestroy(ev);
+    if (ctx->arena) hipFree(ctx->arena);
+    if (ctx->pinned) hipHostFree(ctx->pinned);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_ctx_sync(hcorepp_ctx *ctx) {
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_ctx_event_record(hcorepp_ctx *ctx, int slot) {
+    if (!ctx || slot < 0 || slot >= 16) return fail(HCOREPP_ERR_INVALID, "bad event slot");
+    if (!ctx->events[slot]) HIP_TRY(hipEventCreate(&ctx->events[slot]));
+    HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_ctx_event_elapsed(hcorepp_ctx *ctx, int slot0, int slot1, float *ms) {
+    if (!ctx || !ms || slot0 < 0 || slot0 >= 16 || slot1 < 0 || slot1 >= 16 || !ctx->events[slot0] ||
+        !ctx->events[slot1])
+        return fail(HCOREPP_ERR_INVALID, "bad event slots");
+    HIP_TRY(hipEventSynchronize(ctx->events[slot1]));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->events[slot0], ctx->events[slot1]));
+    return HCOREPP_OK;
+}
+
+void *hcorepp_ctx_stream(hcorepp_ctx *ctx) { return ctx ? (void *) ctx->stream : nullptr; }
+
+size_t hcorepp_ctx_arena_bytes(hcorepp_ctx *ctx) { return ctx ? ctx->arena_cap : 0; }
+
+hcorepp_status hcorepp_malloc(size_t bytes, void **out) {
+    if (!out) return fail(HCOREPP_ERR_INVALID, "out is null");
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return fail(HCOREPP_ERR_NOMEM, "hipMalloc(%zu)", bytes);
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_memcpy(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!bytes) return HCOREPP_OK;
+    if (ctx) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    } else {
+        HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    }
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_memset(hcorepp_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!bytes) return HCOREPP_OK;
+    if (ctx) HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    else HIP_TRY(hipMemset(dst, value, bytes));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_gemm_d(hcorepp_ctx *ctx, double alpha, const hcorepp_tile *A, char opA,
+                              const hcorepp_tile *B, char opB, double beta, hcorepp_tile *C,
+                              const hcorepp_cparams *params, int cholesky, uint64_t *flops, double *sigma_out,
+                              int64_t *n_sigma) {
+    return gemm_plain<double>(ctx, alpha, A, opA, B, opB, beta, C, params, cholesky, flops, sigma_out, n_sigma);
+}
+
+hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile *A, char opA,
+                              const hcorepp_tile *B, char opB, float beta, hcorepp_tile *C,
+                              const hcorepp_cparams *params, int cholesky, uint64_t *flops, double *sigma_out,
+                              int64_t *n_sigma) {
+    return gemm_plain<float>(ctx, alpha, A, opA, B, opB, beta, C, params, cholesky, flops, sigma_out, n_sigma);
+}
+
+size_t hcorepp_gemm_pool_size_d(const hcorepp_tile *A, const hcorepp_tile *B, const hcorepp_tile *C,
+                                const hcorepp_cparams *params) {
+    (void) params;
+    if (!A || !B || !C) return 0;
+    size_t total = 0;
+    const int64_t ra = A->type == HCOREPP_COMPRESSED ? A->rank : A->cols;
+    const int64_t rb = B->type == HCOREPP_COMPRESSED ? B->rank : B->rows;
+    total += (size_t) (ra * rb + ra * B->cols + A->rows * B->cols);
+    if (C->type == HCOREPP_COMPRESSED) {
+        const int64_t arank = std::max(ra, rb);
+        const WsLayout L = ws_layout<double>(C->rows, C->cols, arank + C->rank, C->max_rank);
+        total += (size_t) L.total;
+    }
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *A, const hcorepp_tile *B,
+                                    const hcorepp_tile *C, double alpha, double beta,
+                                    const hcorepp_cparams *params, int precision, hcorepp_batch **out) {
+    if (!ctx || !out || n < 0 || (n > 0 && (!A || !B || !C))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    hcorepp_status st;
+    if ((st = ensure_lds_limits())) return st;
+    if (precision != HCOREPP_PREC_FP64)
+        return fail(HCOREPP_ERR_UNSUPPORTED, "precision form %d not built into this library", precision);
+    hcorepp_cparams p = params ? *params : hcorepp_cparams{1e-9, 0, 1, 0, HCOREPP_SVD_GESDD, 0};
+    using T = double;
+    for (int64_t i = 0; i < n; ++i) {
+        if ((st = validate_tile<T>(&A[i], "A[i]")) || (st = validate_tile<T>(&B[i], "B[i]")) ||
+            (st = validate_tile<T>(&C[i], "C[i]")))
+            return st;
+        if (C[i].type != HCOREPP_COMPRESSED)
+            return fail(HCOREPP_ERR_UNSUPPORTED, "batched form needs compressed C tiles (update %lld)",
+                        (long long) i);
+        if (A[i].type == HCOREPP_DENSE && B[i].type == HCOREPP_DENSE)
+            return fail(HCOREPP_ERR_UNSUPPORTED, "DDC updates are not batched (update %lld)", (long long) i);
+    }
+    // Plan one update: fills `e` (when non-null) with pointers relative to `base`
+    // and returns the bytes of workspace the update needs.
+    auto plan = [&](int64_t i, char *base, BatchEntry<T> *e, double *cost, int64_t *rows) -> size_t {
+        const hcorepp_tile *a = &A[i], *b = &B[i], *c = &C[i];
+        Holder<T> ah[2], bh[2];
+        char aop[2], bop[2];
+        const int na = tile_holders<T>(a, 'N', ah, aop);
+        const int nb = tile_holders<T>(b, 'N', bh, bop);
+        int ntemp = (na + nb == 3) ? 1 : 2;
+        int a_idx = na - 1, b_idx = 0, g = 0;
+        size_t off = 0;
+        while (ntemp > 0) {
+            const Holder<T> ad = ah[a_idx], bd = bh[b_idx];
+            const int64_t mm = ad.rows, nn = bd.cols, kk = std::min(ad.cols, bd.rows);
+            T *dst = (T *) (base + off);
+            if (e) {
+                GemmSpec<T> &gs = e->gemm[g];
+                gs.a = ad.data;
+                gs.b = bd.data;
+                gs.lda = std::max(ad.rows, mm);
+                gs.ldb = std::max(bd.rows, kk);
+                gs.ldc = mm;
+                gs.m = (int32_t) mm;
+                gs.n = (int32_t) nn;
+                gs.k = (int32_t) kk;
+                gs.ta = gs.tb = 0;
+                gs.c = dst;
+            }
+            g++;
+            off += aligned(sizeof(T) * std::max<int64_t>(mm * nn, 1));
+            const Holder<T> th = {mm, nn, mm, dst};
+            if (ntemp == 2 || (ntemp == 1 && a_idx == 0)) {
+                ah[a_idx] = th;
+                b_idx++;
+            } else {
+                bh[b_idx] = th;
+                a_idx--;
+            }
+            ntemp--;
+        }
+        const Holder<T> ad = ah[a_idx], bd = bh[b_idx];
+        const int64_t m = c->rows, nn = c->cols, arank = ad.cols;
+        const int64_t kmax = arank + c->max_rank;
+        const WsLayout L = ws_layout<T>(m, nn, kmax, c->max_rank);
+        if (e) {
+            e->ngemm = g;
+            RecompDesc<T> &d = e->rd;
+            d.afin = ad.data;
+            d.ldaf = ad.ld;
+            d.bfin = bd.data;
+            d.ldbf = arank;
+            d.cdata = (T *) c->data;
+            d.ws = (T *) (base + off);
+            d.m = (int32_t) m;
+            d.n = (int32_t) nn;
+            d.arank = (int32_t) arank;
+            d.max_rank = (int32_t) c->max_rank;
+            d.kmax = (int32_t) kmax;
+            d.truncated_svd = p.truncated_svd;
+            d.fixed_rank = (int32_t) p.fixed_rank;
+            d.alpha = alpha;
+            d.beta = beta;
+            d.accuracy = p.accuracy;
+            d.sigma_out = nullptr;
+        }
+        if (cost) {
+            const double kk = (double) (arank + c->rank);
+            *cost = kk * kk * (double) std::max(m, nn) + 20.0 * kk * kk * kk;
+        }
+        if (rows) *rows = std::max(m, nn);
+        if (e && (ad.rows != m || bd.cols != nn || bd.rows != arank)) return 0;
+        return off + aligned(sizeof(T) * L.total);
+    };
+    std::vector<size_t> ws_off((size_t) n);
+    std::vector<double> cost((size_t) n);
+    size_t ws_total = 0;
+    int64_t max_rows = 1;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t rows = 0;
+        ws_off[(size_t) i] = ws_total;
+        ws_total += plan(i, nullptr, nullptr, &cost[(size_t) i], &rows);
+        max_rows = std::max(max_rows, rows);
+    }
+    const int pb = pick_pb(max_rows, kLdsBytes / (int64_t) sizeof(T));
+    if (pb == 0) return fail(HCOREPP_ERR_UNSUPPORTED, "tile too tall for the LDS panel");
+
+    hcorepp_batch *bt = new (std::nothrow) hcorepp_batch();
+    if (!bt) return fail(HCOREPP_ERR_NOMEM, "host allocation failed");
+    bt->ctx = ctx;
+    bt->precision = precision;
+    bt->n = n;
+    bt->pb = pb;
+    bt->ws_bytes = ws_total;
+    const int occupancy = 2;  // two 80 KB-LDS workgroups per CU
+    bt->grid = (int) std::max<int64_t>(1, std::min<int64_t>(n, (int64_t) std::max(ctx->num_cus, 1) * occupancy));
+    auto cleanup = [&](hcorepp_status s2) {
+        hcorepp_batch_destroy(bt);
+        return s2;
+    };
+    if (hipMalloc(&bt->ws, std::max<size_t>(ws_total, 256)) != hipSuccess)
+        return cleanup(fail(HCOREPP_ERR_NOMEM, "batch workspace of %zu bytes", ws_total));
+    if (hipMalloc(&bt->entries, sizeof(BatchEntry<T>) * (size_t) std::max<int64_t>(n, 1)) != hipSuccess ||
+        hipMalloc(&bt->order, sizeof(int32_t) * (size_t) std::max<int64_t>(n, 1)) != hipSuccess ||
+        hipMalloc(&bt->ranks, sizeof(int64_t) * (size_t) std::max<int64_t>(n, 1)) != hipSuccess ||
+        hipMalloc(&bt->ctr, sizeof(BatchCounters)) != hipSuccess)
+        return cleanup(fail(HCOREPP_ERR_NOMEM, "batch descriptors"));
+    std::vector<BatchEntry<T>> ent((size_t) n);
+    std::vector<int64_t> ranks((size_t) n);
+    for (int64_t i = 0; i < n; ++i) {
+        BatchEntry<T> &e = ent[(size_t) i];
+        memset(&e, 0, sizeof(e));
+        if (plan(i, bt->ws + ws_off[(size_t) i], &e, nullptr, nullptr) == 0)
+            return cleanup(fail(HCOREPP_ERR_INVALID, "operand shapes do not match C (update %lld)", (long long) i));
+        e.rd.crank = bt->ranks + i;
+        ranks[(size_t) i] = C[i].rank;
+    }
+    std::vector<int32_t> order((size_t) n);
+    for (int64_t i = 0; i < n; ++i) order[(size_t) i] = (int32_t) i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+    hipStream_t s = ctx->stream;
+    if (n > 0) {
+        if (hipMemcpyAsync(bt->entries, ent.data(), sizeof(BatchEntry<T>) * n, hipMemcpyHostToDevice, s) !=
+                    hipSuccess ||
+            hipMemcpyAsync(bt->order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(bt->ranks, ranks.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return cleanup(fail(HCOREPP_ERR_HIP, "batch descriptor upload"));
+    }
+    *out = bt;
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_run(hcorepp_batch *b) {
+    if (!b) return fail(HCOREPP_ERR_INVALID, "null batch");
+    if (b->n == 0) return HCOREPP_OK;
+    hipStream_t s = b->ctx->stream;
+    HIP_TRY(launch_reset_counters(s, b->ctr));
+    HIP_TRY(launch_batched<double>(s, (const BatchEntry<double> *) b->entries, b->order, b->n, b->ctr, b->grid,
+                                   kLdsBytes, b->pb));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_capture_graph(hcorepp_batch *b) {
+    if (!b) return fail(HCOREPP_ERR_INVALID, "null batch");
+    hipStream_t s = b->ctx->stream;
+    if (b->graph_exec) {
+        hipGraphExecDestroy(b->graph_exec);
+        b->graph_exec = nullptr;
+    }
+    if (b->graph) {
+        hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hcorepp_status st = hcorepp_batch_run(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(HCOREPP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_launch_graph(hcorepp_batch *b) {
+    if (!b || !b->graph_exec) return fail(HCOREPP_ERR_INVALID, "no captured graph");
+    HIP_TRY(hipGraphLaunch(b->graph_exec, b->ctx->stream));
+    return HCOREPP_OK;
+}
+
+int64_t *hcorepp_batch_rank_dev(hcorepp_batch *b) { return b ? b->ranks : nullptr; }
+
+hcorepp_status hcorepp_batch_ranks(hcorepp_batch *b, int64_t *ranks_out) {
+    if (!b || !ranks_out) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    if (b->n == 0) return HCOREPP_OK;
+    HIP_TRY(hipMemcpyAsync(ranks_out, b->ranks, sizeof(int64_t) * b->n, hipMemcpyDeviceToHost, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_set_ranks(hcorepp_batch *b, const int64_t *ranks) {
+    if (!b || !ranks) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    if (b->n == 0) return HCOREPP_OK;
+    HIP_TRY(hipMemcpyAsync(b->ranks, ranks, sizeof(int64_t) * b->n, hipMemcpyHostToDevice, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps) {
+    if (!b) return fail(HCOREPP_ERR_INVALID, "null batch");
+    HIP_TRY(hipMemcpyAsync(b->ctx->pinned, b->ctr, sizeof(BatchCounters), hipMemcpyDeviceToHost, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    const BatchCounters *c = reinterpret_cast<const BatchCounters *>(b->ctx->pinned);
+    if (max_sweeps) *max_sweeps = (int64_t) c->max_sweeps;
+    if (c->status) return fail(HCOREPP_ERR_NUMERIC, "batch status bits 0x%llx", (unsigned long long) c->status);
+    return HCOREPP_OK;
+}
+
+size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b) { return b ? b->ws_bytes : 0; }
+
+hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b) {
+    if (!b) return HCOREPP_OK;
+    if (b->ctx) hipStreamSynchronize(b->ctx->stream);
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->ws) hipFree(b->ws);
+    if (b->entries) hipFree(b->entries);
+    if (b->order) hipFree(b->order);
+    if (b->ranks) hipFree(b->ranks);
+    if (b->ctr) hipFree(b->ctr);
+    delete b;
+    return HCOREPP_OK;
+}
+
+const char *hcorepp_last_error(void) { return g_last_error.c_str(); }
+
+const char *hcorepp_version(void) { return "hcorepp-amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+// HIP kernels of the TLR recompression engine (gfx950).  Each kernel is a thin
+// wrapper that maps workgroups onto the workgroup-level phases of
+// recompress.hpp / gemm.hpp; the batched persistent kernel runs every phase of
+// one update in one workgroup and pulls updates from a device work queue.
+#include "kernels.hpp"
+
+namespace hcamd {
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_gemm(int M, int N, int Kd, T alpha, const T *A, int64_t lda, int ta,
+                                             const T *B, int64_t ldb, int tb, T beta, T *C, int64_t ldc) {
+    __shared__ T lds[GT_LDS_ELEMS];
+    wg_gemm_tile<T>(M, N, Kd, alpha, A, lda, ta != 0, B, ldb, tb != 0, beta, C, ldc, blockIdx.x, blockIdx.y, lds);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_stack(RecompDesc<T> d) {
+    ph_stack<T>(d, blockIdx.x, gridDim.x);
+}
+
+template <typename T, int PB>
+__global__ void __launch_bounds__(NT) k_qr(RecompDesc<T> d) {
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    Ws<T> w = ws_of(d);
+    const int K = (int) w.hdr[HDR_K];
+    if (blockIdx.x == 0) wg_geqrf<T, PB>(d.m, K, w.us, d.m, w.tau_u, w.t_u, lds);
+    else wg_geqrf<T, PB>(d.n, K, w.vs, d.n, w.tau_v, w.t_v, lds);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_core(RecompDesc<T> d) {
+    ph_core<T>(d, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_svd(RecompDesc<T> d, int lds_elems) {
+    extern __shared__ unsigned char smem_raw[];
+    ph_svd<T>(d, reinterpret_cast<T *>(smem_raw), lds_elems);
+}
+
+template <typename T, int PB>
+__global__ void __launch_bounds__(NT) k_recon(RecompDesc<T> d, int cols_per_wg) {
+    extern __shared__ unsigned char smem_raw[];
+    const int c0 = blockIdx.x * cols_per_wg;
+    ph_recon<T, PB>(d, blockIdx.y, c0, c0 + cols_per_wg, reinterpret_cast<T *>(smem_raw));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_finish(RecompDesc<T> d) {
+    ph_finish<T>(d, blockIdx.x, gridDim.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Ws<T> w = ws_of(d);
+        *d.crank = w.hdr[HDR_RANK];
+    }
+}
+
+__global__ void k_set_i64(int64_t *p, int64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Fill V (n x n, ld n) with the identity (FillIdentityMatrix, reference omp/kernels.cpp:144-151,
+// on a zeroed buffer) and copy U = W (m x n): the DDC ReadjustTile (Compressed.cpp:696-734).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_readjust_ddc(const T *W, int64_t m, int64_t n, T *cdata) {
+    const int64_t tot = m * n + n * n;
+    for (int64_t e = (int64_t) blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t) gridDim.x * NT) {
+        if (e < m * n) {
+            cdata[e] = W[e];
+        } else {
+            const int64_t f = e - m * n;
+            cdata[e] = ((f % n) == (f / n)) ? T(1) : T(0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent kernel.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds) {
+    for (int g = 0; g < e.ngemm; ++g) {
+        const GemmSpec<T> &s = e.gemm[g];
+        wg_gemm<T>(s.m, s.n, s.k, T(1), s.a, s.lda, s.ta, s.b, s.ldb, s.tb, T(0), s.c, s.ldc, lds);
+        __syncthreads();
+    }
+}
+
+template <typename T, int PB>
+__global__ void __launch_bounds__(NT) k_batched(const BatchEntry<T> *entries, const int32_t *order, int64_t n,
+                                                BatchCounters *ctr, int lds_elems) {
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    __shared__ int64_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int64_t) atomicAdd(&ctr->next, 1ULL);
+        __syncthreads();
+        const int64_t item = s_item;
+        __syncthreads();
+        if (item >= n) break;
+        const BatchEntry<T> &e = entries[order[item]];
+        run_temp_gemms<T>(e, lds);
+        const RecompDesc<T> &d = e.rd;
+        ph_stack<T>(d, 0, 1);
+        __syncthreads();
+        Ws<T> w = ws_of(d);
+        const int K = (int) w.hdr[HDR_K];
+        wg_geqrf<T, PB>(d.m, K, w.us, d.m, w.tau_u, w.t_u, lds);
+        __syncthreads();
+        wg_geqrf<T, PB>(d.n, K, w.vs, d.n, w.tau_v, w.t_v, lds);
+        __syncthreads();
+        ph_core<T>(d, 0, 1);
+        __syncthreads();
+        ph_svd<T>(d, lds, lds_elems);
+        __syncthreads();
+        const int r = (int) w.hdr[HDR_RANK];
+        ph_recon<T, PB>(d, 0, 0, r, lds);
+        __syncthreads();
+        ph_recon<T, PB>(d, 1, 0, r, lds);
+        __syncthreads();
+        ph_finish<T>(d, 0, 1);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            *d.crank = r;
+            const int64_t st = w.hdr[HDR_STATUS];
+            if (st) atomicOr(&ctr->status, (unsigned long long) st);
+            atomicMax(&ctr->max_sweeps, (unsigned long long) w.hdr[HDR_SWEEPS]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_counters(BatchCounters *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->next = 0;
+        c->status = 0;
+        c->max_sweeps = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launch wrappers
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B,
+                       int64_t ldb, bool tb, T beta, T *C, int64_t ldc) {
+    if (M <= 0 || N <= 0) return hipSuccess;
+    dim3 grid((M + GT_M - 1) / GT_M, (N + GT_N - 1) / GT_N);
+    hipLaunchKernelGGL(k_gemm<T>, grid, dim3(NT), 0, s, M, N, Kd, alpha, A, lda, (int) ta, B, ldb, (int) tb, beta,
+                       C, ldc);
+    return hipGetLastError();
+}
+
+int pick_pb(int64_t rows, int64_t lds_elems) {
+    const int pbs[4] = {16, 8, 4, 2};
+    for (int i = 0; i < 4; ++i) {
+        const int pb = pbs[i];
+        if (rows * pb + 2 * pb * pb + NWAVES * pb + 64 <= lds_elems) return pb;
+    }
+    return 0;
+}
+
+template <typename T, int PB>
+static hipError_t launch_recomp_chain_pb(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int phase,
+                                         int r) {
+    if (phase == 0) {
+        hipLaunchKernelGGL((k_qr<T, PB>), dim3(2), dim3(NT), lds_bytes, s, d);
+    } else {
+        const int cols = 8;
+        const int nch = (r + cols - 1) / cols;
+        hipLaunchKernelGGL((k_recon<T, PB>), dim3(nch > 0 ? nch : 1, 2), dim3(NT), lds_bytes, s, d, cols);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb) {
+    const int64_t m = d.m, n = d.n;
+    int nwg = (int) ((m * K + NT - 1) / NT);
+    if (nwg > 1024) nwg = 1024;
+    if (nwg < 1) nwg = 1;
+    hipLaunchKernelGGL(k_stack<T>, dim3(nwg), dim3(NT), 0, s, d);
+    hipError_t e;
+    switch (pb) {
+        case 16: e = launch_recomp_chain_pb<T, 16>(s, d, lds_bytes, 0, 0); break;
+        case 8: e = launch_recomp_chain_pb<T, 8>(s, d, lds_bytes, 0, 0); break;
+        case 4: e = launch_recomp_chain_pb<T, 4>(s, d, lds_bytes, 0, 0); break;
+        default: e = launch_recomp_chain_pb<T, 2>(s, d, lds_bytes, 0, 0); break;
+    }
+    if (e != hipSuccess) return e;
+    const int64_t mu = m < K ? m : K, mv = n < K ? n : K;
+    int nwc = (int) ((mu * mv + NT - 1) / NT);
+    if (nwc > 1024) nwc = 1024;
+    if (nwc < 1) nwc = 1;
+    hipLaunchKernelGGL(k_core<T>, dim3(nwc), dim3(NT), 0, s, d);
+    const int lds_elems = (int) (lds_bytes / sizeof(T));
+    hipLaunchKernelGGL(k_svd<T>, dim3(1), dim3(NT), lds_bytes, s, d, lds_elems);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_recomp_post(hipStream_t s, const RecompDesc<T> &d, int64_t r, int64_t lds_bytes, int pb) {
+    hipError_t e;
+    switch (pb) {
+        case 16: e = launch_recomp_chain_pb<T, 16>(s, d, lds_bytes, 1, (int) r); break;
+        case 8: e = launch_recomp_chain_pb<T, 8>(s, d, lds_bytes, 1, (int) r); break;
+        case 4: e = launch_recomp_chain_pb<T, 4>(s, d, lds_bytes, 1, (int) r); break;
+        default: e = launch_recomp_chain_pb<T, 2>(s, d, lds_bytes, 1, (int) r); break;
+    }
+    if (e != hipSuccess) return e;
+    int nwg = (int) ((r * d.n + NT - 1) / NT);
+    if (nwg > 1024) nwg = 1024;
+    if (nwg < 1) nwg = 1;
+    hipLaunchKernelGGL(k_finish<T>, dim3(nwg), dim3(NT), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_i64(hipStream_t s, int64_t *p, int64_t v) {
+    hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_readjust_ddc(hipStream_t s, const T *W, int64_t m, int64_t n, T *cdata) {
+    int nwg = (int) ((m * n + n * n + NT - 1) / NT);
+    if (nwg > 1024) nwg = 1024;
+    hipLaunchKernelGGL(k_readjust_ddc<T>, dim3(nwg), dim3(NT), 0, s, W, m, n, cdata);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, c);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,
+                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb) {
+    const int lds_elems = (int) (lds_bytes / sizeof(T));
+    switch (pb) {
+        case 16: hipLaunchKernelGGL((k_batched<T, 16>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
+        case 8: hipLaunchKernelGGL((k_batched<T, 8>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
+        case 4: hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
+        default: hipLaunchKernelGGL((k_batched<T, 2>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
+    // Allow the dynamic-LDS kernels to use more than the default 64 KB.
+    const void *fns[] = {
+            (const void *) k_qr<T, 16>, (const void *) k_qr<T, 8>, (const void *) k_qr<T, 4>,
+            (const void *) k_qr<T, 2>, (const void *) k_svd<T>, (const void *) k_recon<T, 16>,
+            (const void *) k_recon<T, 8>, (const void *) k_recon<T, 4>, (const void *) k_recon<T, 2>,
+            (const void *) k_batched<T, 16>, (const void *) k_batched<T, 8>, (const void *) k_batched<T, 4>,
+            (const void *) k_batched<T, 2>,
+    };
+    for (const void *f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+#define HCAMD_INSTANTIATE(T)                                                                                     \
+    template hipError_t launch_gemm<T>(hipStream_t, int, int, int, T, const T *, int64_t, bool, const T *,      \
+                                       int64_t, bool, T, T *, int64_t);                                          \
+    template hipError_t launch_recomp_pre<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);         \
+    template hipError_t launch_recomp_post<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);        \
+    template hipError_t launch_readjust_ddc<T>(hipStream_t, const T *, int64_t, int64_t, T *);                   \
+    template hipError_t launch_batched<T>(hipStream_t, const BatchEntry<T> *, const int32_t *, int64_t,          \
+                                          BatchCounters *, int, int64_t, int);                                   \
+    template hipError_t set_kernel_lds_limits<T>(int64_t);
+
+HCAMD_INSTANTIATE(double)
+HCAMD_INSTANTIATE(float)
+
+}  // namespace hcamd

@@ -1,0 +1,62 @@
+// Declarations shared between the kernels (kernels.hip) and the host side of
+// the C-ABI (api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+#include "gemm.hpp"
+#include "recompress.hpp"
+
+namespace hcamd {
+
+// One temporary factor product W = op(A) * op(B) (alpha 1, beta 0), as the
+// reference's temporary DenseTile GEMMs (src/api/HCore.cpp:176-264).
+template <typename T>
+struct GemmSpec {
+    const T *a;
+    const T *b;
+    T *c;
+    int64_t lda, ldb, ldc;
+    int32_t m, n, k;
+    int32_t ta, tb;
+    int32_t pad;
+};
+
+// One update of a batch: up to two temporary products then the recompression.
+template <typename T>
+struct BatchEntry {
+    GemmSpec<T> gemm[2];
+    int32_t ngemm;
+    int32_t pad;
+    RecompDesc<T> rd;
+};
+
+struct BatchCounters {
+    unsigned long long next;
+    unsigned long long status;
+    unsigned long long max_sweeps;
+    unsigned long long pad;
+};
+
+int pick_pb(int64_t rows, int64_t lds_elems);
+
+template <typename T>
+hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B,
+                       int64_t ldb, bool tb, T beta, T *C, int64_t ldc);
+template <typename T>
+hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb);
+template <typename T>
+hipError_t launch_recomp_post(hipStream_t s, const RecompDesc<T> &d, int64_t r, int64_t lds_bytes, int pb);
+hipError_t launch_set_i64(hipStream_t s, int64_t *p, int64_t v);
+template <typename T>
+hipError_t launch_readjust_ddc(hipStream_t s, const T *W, int64_t m, int64_t n, T *cdata);
+hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c);
+template <typename T>
+hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,
+                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb);
+template <typename T>
+hipError_t set_kernel_lds_limits(int64_t lds_bytes);
+
+}  // namespace hcamd

@@ -1,0 +1,91 @@
+"""Synthetic workloads for the TLR tile GEMM + recompression path.
+
+* ``covariance_points`` / ``covariance_matrix``: points uniform in the unit
+  square/cube from a seed, sorted along a Morton (Z-order) curve, and the
+  exponential covariance exp(-|x_i - x_j| / ell) (+ nugget * I).
+* ``lowrank_factors``: U = orth(G_{m x r}), V = diag(decay^i) orth(G_{n x r})^T,
+  i.e. U has orthonormal columns and V carries the singular values, the
+  reference's compressed-tile convention (src/operators/concrete/Compressed.cpp:121-135).
+* ``batch_ccc``: independent CCC updates with ranks drawn log-uniform.
+
+All generation is host-side numpy with ``numpy.random.default_rng(seed)``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BASE_SEED = 20261202
+
+
+def _spread_bits(x: np.ndarray, dims: int) -> np.ndarray:
+    x = x.astype(np.uint64)
+    out = np.zeros_like(x)
+    for b in range(21 if dims == 3 else 32):
+        out |= ((x >> np.uint64(b)) & np.uint64(1)) << np.uint64(dims * b)
+    return out
+
+
+def morton_order(points: np.ndarray) -> np.ndarray:
+    """Permutation sorting ``points`` (n x d in [0,1)^d, d in {2,3}) along a Morton curve."""
+    n, d = points.shape
+    bits = 21 if d == 3 else 31
+    q = np.clip(np.floor(points * (1 << bits)), 0, (1 << bits) - 1).astype(np.uint64)
+    code = np.zeros(n, dtype=np.uint64)
+    for k in range(d):
+        code |= _spread_bits(q[:, k], d) << np.uint64(k)
+    return np.argsort(code, kind="stable")
+
+
+def covariance_points(n: int, dims: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    p = rng.random((n, dims))
+    return p[morton_order(p)]
+
+
+def covariance_matrix(points: np.ndarray, ell: float, nugget: float = 0.0, rows=None, cols=None) -> np.ndarray:
+    """exp(-|x_i - x_j| / ell) (+ nugget on the diagonal) for row/col index ranges."""
+    rows = slice(None) if rows is None else rows
+    cols = slice(None) if cols is None else cols
+    X, Y = points[rows], points[cols]
+    d2 = np.zeros((X.shape[0], Y.shape[0]))
+    for k in range(points.shape[1]):
+        d2 += (X[:, k:k + 1] - Y[None, :, k]) ** 2
+    A = np.exp(-np.sqrt(d2) / ell)
+    if nugget:
+        ri = np.arange(points.shape[0])[rows]
+        ci = np.arange(points.shape[0])[cols]
+        A[ri[:, None] == ci[None, :]] += nugget
+    return np.asfortranarray(A)
+
+
+def _orth(rng, m, r):
+    q, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    return q
+
+
+def lowrank_factors(rng, m, n, r, decay=0.85, scale=1.0):
+    """U (m x r, orthonormal columns), V (r x n) = diag(scale*decay^i) Q^T."""
+    u = _orth(rng, m, r)
+    q = _orth(rng, n, r)
+    s = scale * decay ** np.arange(r)
+    v = (s[:, None] * q.T)
+    return np.asfortranarray(u), np.asfortranarray(v)
+
+
+def loguniform_ranks(rng, n, lo=8, hi=192):
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n))).astype(np.int64)
+
+
+def batch_ccc(n: int, nb: int, seed: int, lo=8, hi=192, decay=0.85):
+    """Independent CCC updates: returns list of dicts with (au, av, bu, bv, cu, cv)."""
+    rng = np.random.default_rng(seed)
+    ra = loguniform_ranks(rng, n, lo, hi)
+    rb = loguniform_ranks(rng, n, lo, hi)
+    rc = loguniform_ranks(rng, n, lo, hi)
+    out = []
+    for i in range(n):
+        au, av = lowrank_factors(rng, nb, nb, int(ra[i]), decay)
+        bu, bv = lowrank_factors(rng, nb, nb, int(rb[i]), decay)
+        cu, cv = lowrank_factors(rng, nb, nb, int(rc[i]), decay)
+        out.append(dict(au=au, av=av, bu=bu, bv=bv, cu=cu, cv=cv))
+    return out

@@ -1,0 +1,177 @@
+/*
+ * hcorepp_amd.h -- C-ABI of the MI355X-native TLR tile GEMM + recompression engine.
+ *
+ * This is the drop-in boundary: plain pointers and sizes, no C++ or torch types.
+ * The C++ surface in include/hcorepp/ (HCore<T>::Gemm, DenseTile<T>,
+ * CompressedTile<T>, CompressionParameters, RunContext, MemoryUnit) forwards to
+ * these entry points.  Each declaration names the reference interface it
+ * replaces (paths relative to the ecrc/hcorepp source tree).
+ *
+ * Conventions
+ *  - All matrices are column-major (blas::Layout::ColMajor).
+ *  - Tile data pointers are device pointers (HBM).  A compressed tile's buffer
+ *    holds [U rows x max_rank (ld rows) | V max_rank x cols (ld rank)]
+ *    (reference src/operators/concrete/Compressed.cpp:175-185).
+ *  - Functions return HCOREPP_OK (0) or a negative hcorepp_status; nothing
+ *    throws across the boundary.  hcorepp_last_error() describes the last
+ *    failure on the calling thread.
+ *  - One hcorepp_ctx per host thread.  Calls on different contexts are
+ *    thread-safe; calls sharing a context must be serialised by the caller.
+ */
+#ifndef HCOREPP_AMD_H
+#define HCOREPP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    HCOREPP_OK = 0,
+    HCOREPP_ERR_INVALID = -1,      /* bad argument (null pointer, inconsistent sizes) */
+    HCOREPP_ERR_UNSUPPORTED = -2,  /* combination the reference does not define (see hcorepp_gemm_d) */
+    HCOREPP_ERR_CAPACITY = -3,     /* output does not fit the tile's max_rank capacity */
+    HCOREPP_ERR_HIP = -4,          /* HIP runtime error */
+    HCOREPP_ERR_NUMERIC = -5,      /* Jacobi SVD did not converge or NaN/Inf encountered */
+    HCOREPP_ERR_NOMEM = -6         /* device allocation failed */
+} hcorepp_status;
+
+/* TileType (reference include/hcorepp/operators/interface/Tile.hpp:21-24). */
+typedef enum { HCOREPP_DENSE = 0, HCOREPP_COMPRESSED = 1 } hcorepp_tile_type;
+
+/* SVD implementation selector (reference include/hcorepp/common/Definitions.hpp:101-104).
+ * The device path uses one-sided Jacobi for both; the value is accepted for parity. */
+typedef enum { HCOREPP_SVD_GESVD = 0, HCOREPP_SVD_GESDD = 1 } hcorepp_svd_type;
+
+/* Precision form of a batch (north-star forms). */
+typedef enum {
+    HCOREPP_PREC_FP64 = 0,   /* everything in FP64 */
+    HCOREPP_PREC_MIXED = 1   /* factor products + reconstruction products in FP32 (FP32 accumulate)
+                                on the terms whose rounding stays below tolerance; QR, SVD, rank in FP64 */
+} hcorepp_precision;
+
+/* TileMetadata + data pointer (reference include/hcorepp/operators/interface/Tile.hpp:30-52).
+ * `rank` is written back by the callee for compressed outputs. */
+typedef struct {
+    int64_t rows;      /* mNumOfRows */
+    int64_t cols;      /* mNumOfCols */
+    int64_t rank;      /* mMatrixRank (compressed tiles) */
+    int64_t max_rank;  /* mMaxRank: capacity of the U/V slots */
+    int64_t ld;        /* mLeadingDimension (dense tiles) */
+    int32_t layout;    /* 0 = ColMajor (the only layout the reference's kernels implement) */
+    int32_t type;      /* hcorepp_tile_type */
+    void *data;        /* device pointer */
+} hcorepp_tile;
+
+/* CompressionParameters (reference include/hcorepp/operators/helpers/CompressionParameters.hpp:44-46).
+ * trmm / ungqr / svd_type select LAPACK call sequences in the reference; the device path computes
+ * the same truncated SVD regardless and accepts them for parity. */
+typedef struct {
+    double accuracy;        /* absolute truncation threshold (default 1e-4) */
+    int32_t trmm;           /* default 0 */
+    int32_t ungqr;          /* default 1 */
+    int32_t truncated_svd;  /* default 0: relative threshold accuracy*sigma_0 when 1 */
+    int32_t svd_type;       /* hcorepp_svd_type, default GESDD */
+    int64_t fixed_rank;     /* default 0: truncate to this rank when > 0 */
+} hcorepp_cparams;
+
+/* Opaque execution context: one HIP stream, one growable device arena, pinned host mirror.
+ * Replaces RunContext (reference src/kernels/cuda/RunContext.cpp:11-78) and the pool
+ * MemoryUnit (reference include/hcorepp/data-units/memory-handlers/pool/MemoryHandler.hpp:22-157). */
+typedef struct hcorepp_ctx hcorepp_ctx;
+
+/* Replaces RunContext() + MemoryHandler::Initialize (reference src/kernels/cuda/RunContext.cpp:11-20,
+ * src/data-units/memory-handlers/pool/MemoryHandler.cpp:23-35).  arena_bytes may be 0 (grown on demand). */
+hcorepp_status hcorepp_ctx_create(int device, size_t arena_bytes, hcorepp_ctx **out);
+hcorepp_status hcorepp_ctx_destroy(hcorepp_ctx *ctx);
+/* RunContext::Sync (reference src/kernels/cuda/RunContext.cpp:48-50). */
+hcorepp_status hcorepp_ctx_sync(hcorepp_ctx *ctx);
+/* Returns the context's hipStream_t (as void*) so callers can order their own work on it. */
+void *hcorepp_ctx_stream(hcorepp_ctx *ctx);
+/* Bytes currently reserved by the context arena. */
+size_t hcorepp_ctx_arena_bytes(hcorepp_ctx *ctx);
+
+/* Device memory helpers (reference include/hcorepp/kernels/memory.hpp:46-121:
+ * AllocateArray, DestroyArray, Memcpy (cudaMemcpyDefault semantics), Memset). */
+hcorepp_status hcorepp_malloc(size_t bytes, void **out);
+hcorepp_status hcorepp_free(void *ptr);
+hcorepp_status hcorepp_memcpy(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes);
+hcorepp_status hcorepp_memset(hcorepp_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* Stream-ordered timing on the context stream: record event `slot` (0..15) and read the
+ * elapsed milliseconds between two recorded slots (synchronises on the second). */
+hcorepp_status hcorepp_ctx_event_record(hcorepp_ctx *ctx, int slot);
+hcorepp_status hcorepp_ctx_event_elapsed(hcorepp_ctx *ctx, int slot0, int slot1, float *ms);
+
+/* ---------------------------------------------------------------------------
+ * One tile update C = alpha * op(A) * op(B) + beta * C.
+ * Replaces hcorepp::api::HCore<T>::Gemm (reference include/hcorepp/api/HCore.hpp:41-46,
+ * src/api/HCore.cpp:22-344) including DenseTile::Gemm (src/operators/concrete/Dense.cpp:46-108)
+ * and CompressedTile::Gemm (src/operators/concrete/Compressed.cpp:208-694).
+ *  - All 8 dense/compressed combinations.  Compressed C requires opA = opB = 'N' (the
+ *    reference's CompressedTile::Gemm ignores the ops, Compressed.cpp:331-380); other ops
+ *    return HCOREPP_ERR_UNSUPPORTED.  Dense C supports 'N' and 'T'.
+ *  - cholesky != 0 selects the reference's aCholesky operand plumbing (HCore.cpp:160-174):
+ *    for compressed A and B with opA='N', opB='T' it computes C = alpha*A*B^T + beta*C.
+ *  - Plain form: the recompression runs as a chain of launches on the context stream and the
+ *    new rank is read back with one stream synchronisation, as the reference does
+ *    (src/kernels/cuda/CudaKernels.cu:656-697).  C->rank is updated on return.
+ *  - flops (may be NULL) is incremented with the reference's own flop model.
+ *  - sigma_out (may be NULL, host pointer) receives the sorted core singular values
+ *    (size_s = min(rows, cols, K) entries) for compressed C; *n_sigma their count. */
+hcorepp_status hcorepp_gemm_d(hcorepp_ctx *ctx, double alpha, const hcorepp_tile *A, char opA,
+                              const hcorepp_tile *B, char opB, double beta, hcorepp_tile *C,
+                              const hcorepp_cparams *params, int cholesky, uint64_t *flops,
+                              double *sigma_out, int64_t *n_sigma);
+/* FP32 instantiation (reference HCOREPP_INSTANTIATE_CLASS, include/hcorepp/common/Definitions.hpp:10-13). */
+hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile *A, char opA,
+                              const hcorepp_tile *B, char opB, float beta, hcorepp_tile *C,
+                              const hcorepp_cparams *params, int cholesky, uint64_t *flops,
+                              double *sigma_out, int64_t *n_sigma);
+
+/* Workspace (elements of T) one update needs: reference HCore<T>::CalculateMemoryPoolSize
+ * (src/api/HCore.cpp:417-480).  The engine's layout differs, so the number is this engine's. */
+size_t hcorepp_gemm_pool_size_d(const hcorepp_tile *A, const hcorepp_tile *B, const hcorepp_tile *C,
+                                const hcorepp_cparams *params);
+
+/* ---------------------------------------------------------------------------
+ * Batched variable-rank form (no counterpart call in the reference: it replaces the
+ * reference drivers' loop of HCore::Gemm calls over independent C tiles,
+ * examples/matrix_multiplication/omp_main.cpp:112-126 and
+ * par_fixed_rank_streams_main.cpp:104-137).  Each update is C_i = alpha*A_i*B_i + beta*C_i
+ * with opA = opB = 'N'; every C_i must be distinct.  Ranks of C tiles live on the device
+ * (rank_dev) and are updated in place; host descriptors are not touched by hcorepp_batch_run.
+ * ------------------------------------------------------------------------- */
+typedef struct hcorepp_batch hcorepp_batch;
+
+hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *A, const hcorepp_tile *B,
+                                    const hcorepp_tile *C, double alpha, double beta,
+                                    const hcorepp_cparams *params, int precision, hcorepp_batch **out);
+/* Enqueue one pass over the batch on the context stream (capturable: no host sync, no allocation). */
+hcorepp_status hcorepp_batch_run(hcorepp_batch *b);
+/* Capture hcorepp_batch_run into a hipGraph; later calls of hcorepp_batch_launch_graph replay it. */
+hcorepp_status hcorepp_batch_capture_graph(hcorepp_batch *b);
+hcorepp_status hcorepp_batch_launch_graph(hcorepp_batch *b);
+/* Device array (int64, n entries) holding the current rank of every C tile. */
+int64_t *hcorepp_batch_rank_dev(hcorepp_batch *b);
+/* Copy ranks to host (synchronises the context stream). */
+hcorepp_status hcorepp_batch_ranks(hcorepp_batch *b, int64_t *ranks_out);
+/* Reset the device ranks of the C tiles to the values given (host array). */
+hcorepp_status hcorepp_batch_set_ranks(hcorepp_batch *b, const int64_t *ranks);
+/* Worst status over the last pass (synchronises): 0, or HCOREPP_ERR_NUMERIC. Also max Jacobi sweeps. */
+hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps);
+/* Bytes of device workspace the batch holds. */
+size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b);
+hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b);
+
+/* Human-readable description of the last error on this thread. */
+const char *hcorepp_last_error(void);
+/* Library build/version string. */
+const char *hcorepp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HCOREPP_AMD_H */
