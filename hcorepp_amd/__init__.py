@@ -17,8 +17,10 @@ is missing -- there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -51,14 +53,31 @@ class _CParams(C.Structure):
 # Every symbol include/hcorepp_amd.h declares.
 EXPORTED = [
     "hcorepp_ctx_create", "hcorepp_ctx_destroy", "hcorepp_ctx_sync", "hcorepp_ctx_stream",
-    "hcorepp_ctx_arena_bytes", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy", "hcorepp_memset",
+    "hcorepp_ctx_arena_bytes", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
+    "hcorepp_memcpy_async", "hcorepp_memset",
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
-    "hcorepp_batch_workspace_bytes", "hcorepp_batch_destroy", "hcorepp_last_error", "hcorepp_version",
+    "hcorepp_batch_workspace_bytes", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
+    "hcorepp_version",
 ]
 
 _lib = None
+_live = weakref.WeakSet()   # device-owning objects, released before the HIP runtime shuts down
+_shutting_down = False
+
+
+@atexit.register
+def _release_all():
+    """Free device objects while the HIP runtime is still alive; later finalizers become no-ops."""
+    global _shutting_down
+    order = {"Batch": 0, "DeviceBuffer": 1, "RunContext": 2}
+    for obj in sorted(list(_live), key=lambda o: order.get(type(o).__name__, 1)):
+        try:
+            obj._release()
+        except Exception:
+            pass
+    _shutting_down = True
 
 
 def lib():
@@ -79,6 +98,7 @@ def lib():
     L.hcorepp_ctx_arena_bytes.restype = C.c_size_t
     L.hcorepp_memcpy.argtypes = [P, P, P, C.c_size_t]
     L.hcorepp_memset.argtypes = [P, P, C.c_int, C.c_size_t]
+    L.hcorepp_memcpy_async.argtypes = [P, P, P, C.c_size_t]
     L.hcorepp_malloc.argtypes = [C.c_size_t, C.POINTER(P)]
     L.hcorepp_free.argtypes = [P]
     L.hcorepp_ctx_event_record.argtypes = [P, C.c_int]
@@ -103,6 +123,8 @@ def lib():
     L.hcorepp_batch_status.argtypes = [P, C.POINTER(I64)]
     L.hcorepp_batch_workspace_bytes.argtypes = [P]
     L.hcorepp_batch_workspace_bytes.restype = C.c_size_t
+    L.hcorepp_debug_workspace.argtypes = [P, P, C.c_size_t, P]
+    L.hcorepp_debug_workspace.restype = I64
     L.hcorepp_last_error.restype = C.c_char_p
     L.hcorepp_version.restype = C.c_char_p
     _lib = L
@@ -122,6 +144,7 @@ class DeviceBuffer:
         h = C.c_void_p()
         _check(lib().hcorepp_malloc(max(self.nbytes, 1), C.byref(h)))
         self.ptr = h.value
+        _live.add(self)
 
     @staticmethod
     def from_numpy(a: np.ndarray) -> "DeviceBuffer":
@@ -152,16 +175,61 @@ class DeviceBuffer:
     def data_ptr(self) -> int:
         return self.ptr
 
-    def free(self):
-        if getattr(self, "ptr", None):
+    def _release(self):
+        if getattr(self, "ptr", None) and not _shutting_down:
             lib().hcorepp_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    free = _release
 
     def __del__(self):
         try:
-            self.free()
+            self._release()
         except Exception:
             pass
+
+
+class BufferView:
+    """A byte range [offset, offset + nbytes) of a DeviceBuffer (shares its lifetime)."""
+
+    def __init__(self, parent: DeviceBuffer, offset: int, nbytes: int):
+        assert offset + nbytes <= parent.nbytes
+        self.parent, self.offset, self.nbytes = parent, int(offset), int(nbytes)
+
+    @property
+    def ptr(self) -> int:
+        return self.parent.ptr + self.offset
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def upload(self, a: np.ndarray, offset_bytes: int = 0):
+        self.parent.upload(a, self.offset + offset_bytes)
+
+    def download(self, dtype, count: int | None = None, offset_bytes: int = 0) -> np.ndarray:
+        itemsize = np.dtype(dtype).itemsize
+        if count is None:
+            count = (self.nbytes - offset_bytes) // itemsize
+        return self.parent.download(dtype, count, self.offset + offset_bytes)
+
+
+class TileArena:
+    """One device allocation holding many tiles back to back (256-byte aligned), so that a
+    whole set of tiles can be saved/restored with a single device-to-device copy."""
+
+    def __init__(self, nbytes: int):
+        self.buf = DeviceBuffer(nbytes)
+        self.off = 0
+
+    def take(self, nbytes: int) -> BufferView:
+        off = (self.off + 255) // 256 * 256
+        v = BufferView(self.buf, off, nbytes)
+        self.off = off + nbytes
+        return v
+
+    @staticmethod
+    def compressed_bytes(rows, cols, max_rank, dtype=np.float64):
+        return (rows + cols) * max(int(max_rank), 1) * np.dtype(dtype).itemsize + 256
 
 
 class CompressionParameters:
@@ -187,6 +255,7 @@ class RunContext:
         h = C.c_void_p()
         _check(lib().hcorepp_ctx_create(device, arena_bytes, C.byref(h)))
         self.h = h
+        _live.add(self)
 
     def sync(self):
         _check(lib().hcorepp_ctx_sync(self.h))
@@ -206,14 +275,16 @@ class RunContext:
         _check(lib().hcorepp_ctx_event_elapsed(self.h, slot0, slot1, C.byref(ms)))
         return float(ms.value)
 
-    def close(self):
-        if getattr(self, "h", None):
+    def _release(self):
+        if getattr(self, "h", None) and not _shutting_down:
             lib().hcorepp_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    close = _release
 
     def __del__(self):
         try:
-            self.close()
+            self._release()
         except Exception:
             pass
 
@@ -252,7 +323,7 @@ class CompressedTile(_TileBase):
     [U rows x max_rank | V max_rank x cols] (src/operators/concrete/Compressed.cpp:19-47,175-185)."""
     type = COMPRESSED
 
-    def __init__(self, u, v, max_rank=None, device=0, dtype=np.float64):
+    def __init__(self, u, v, max_rank=None, device=0, dtype=np.float64, arena: "TileArena | None" = None):
         u = np.asarray(u, dtype=dtype)
         v = np.asarray(v, dtype=dtype)
         self.rows, r = u.shape
@@ -267,7 +338,11 @@ class CompressedTile(_TileBase):
         host[: self.rows * r] = u.reshape(-1, order="F")
         off = self.rows * self.max_rank
         host[off: off + r * self.cols] = v.reshape(-1, order="F")
-        self.buf = DeviceBuffer.from_numpy(host)
+        if arena is None:
+            self.buf = DeviceBuffer.from_numpy(host)
+        else:
+            self.buf = arena.take(host.nbytes)
+            self.buf.upload(host)
 
     def factors(self):
         h = self.buf.download(self.dtype)
@@ -331,6 +406,7 @@ class Batch:
         _check(L.hcorepp_batch_create(ctx.h, n, TA, TB, TC, float(alpha), float(beta), C.byref(p),
                                       int(precision), C.byref(h)))
         self.h = h
+        _live.add(self)
 
     def run(self):
         _check(lib().hcorepp_batch_run(self.h))
@@ -364,13 +440,15 @@ class Batch:
             c.rank = int(r)
         return rk
 
-    def close(self):
-        if getattr(self, "h", None):
+    def _release(self):
+        if getattr(self, "h", None) and not _shutting_down:
             lib().hcorepp_batch_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    close = _release
 
     def __del__(self):
         try:
-            self.close()
+            self._release()
         except Exception:
             pass

@@ -54,6 +54,9 @@ struct hcorepp_ctx {
     size_t pinned_sigma_cap = 0;
     int num_cus = 0;
     hipEvent_t events[16] = {};
+    // last plain recompression (diagnostics: hcorepp_debug_workspace)
+    void *last_ws = nullptr;
+    int64_t last_info[16] = {};
 };
 
 namespace {
@@ -318,6 +321,10 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
                     (long long) n);
     const int pb = pick_pb(std::max(m, n), kLdsBytes / (int64_t) sizeof(T));
     if (pb == 0) return fail(HCOREPP_ERR_UNSUPPORTED, "tile too tall for the LDS panel (rows %lld)", (long long) m);
+    if (std::min(m, K) > JACOBI_ROWS_MAX)
+        return fail(HCOREPP_ERR_UNSUPPORTED, "stacked rank %lld exceeds the core SVD limit %d", (long long) K,
+                    JACOBI_ROWS_MAX);
+    if (arank < 1) return fail(HCOREPP_ERR_INVALID, "the A factor has no columns");
     const WsLayout L = ws_layout<T>(m, n, K, C->max_rank);
     RecompDesc<T> d{};
     d.afin = ad.data;
@@ -339,6 +346,12 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
     d.accuracy = p.accuracy;
     d.sigma_out = (double *) arena_take(ctx, sizeof(double) * (size_t) std::max<int64_t>(K, 1));
     HIP_TRY(launch_set_i64(s, d.crank, rc));
+    ctx->last_ws = d.ws;
+    {
+        const int64_t info[16] = {m, n, K, K, C->max_rank, (int64_t) sizeof(T), L.total, L.hdr, L.us, L.vs,
+                                  L.y, L.t_u, L.t_y, L.j, L.v1, L.vtmp};
+        for (int i = 0; i < 16; ++i) ctx->last_info[i] = info[i];
+    }
     HIP_TRY(launch_recomp_pre<T>(s, d, K, kLdsBytes, pb));
     // The plain form reads the new rank back with one stream synchronisation,
     // as the reference does (src/kernels/cuda/CudaKernels.cu:656-697).
@@ -350,6 +363,7 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
     const int64_t status = ctx->pinned[HDR_STATUS];
     if (status & STATUS_NAN) return fail(HCOREPP_ERR_NUMERIC, "NaN/Inf in the core SVD");
     if (status & STATUS_NOCONV) return fail(HCOREPP_ERR_NUMERIC, "core Jacobi SVD did not converge");
+    if (status & STATUS_TOO_BIG) return fail(HCOREPP_ERR_CAPACITY, "stacked rank exceeds the workspace");
     if (r < 1 || r > C->max_rank) return fail(HCOREPP_ERR_NUMERIC, "invalid rank %lld", (long long) r);
     HIP_TRY(launch_recomp_post<T>(s, d, r, kLdsBytes, pb));
     if (sigma_out && size_s > 0) {
@@ -484,6 +498,12 @@ hcorepp_status hcorepp_memcpy(hcorepp_ctx *ctx, void *dst, const void *src, size
     } else {
         HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
     }
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_memcpy_async(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
     return HCOREPP_OK;
 }
 
@@ -761,6 +781,19 @@ hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b) {
     if (b->ctr) hipFree(b->ctr);
     delete b;
     return HCOREPP_OK;
+}
+
+// Diagnostics: copy the workspace of the last plain recompression on this context to `host`
+// (at most `cap` bytes) and its layout to info[16] = {m, n, K, kmax, max_rank, sizeof(T), total,
+// hdr, us, vs, y, t_u, t_y, j, v1, vtmp} (offsets in elements).  Returns bytes copied or < 0.
+int64_t hcorepp_debug_workspace(hcorepp_ctx *ctx, void *host, size_t cap, int64_t *info) {
+    if (!ctx || !ctx->last_ws) return -1;
+    for (int i = 0; i < 16; ++i) info[i] = ctx->last_info[i];
+    size_t bytes = (size_t) (ctx->last_info[6] * ctx->last_info[5]);
+    if (bytes > cap) bytes = cap;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -2;
+    if (hipMemcpy(host, ctx->last_ws, bytes, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+    return (int64_t) bytes;
 }
 
 const char *hcorepp_last_error(void) { return g_last_error.c_str(); }

@@ -59,57 +59,47 @@ __device__ __forceinline__ T wg_sum(T v, T *red) {
 }
 
 // Reduce NV independent partial sums held by every lane of a wave so that,
-// on return, lane l (l < NV) holds the full wave sum of value index l.
-// Uses the halving "transpose" scheme: log2(64) exchange rounds with a total of
-// NV-1 + (6 - log2 NV) shuffles instead of NV * 6.
+// on return, every lane holds the full wave sum of one value index:
+// index = lane >> (6 - log2 NV) (lane bit 5 is the index MSB).  Each halving
+// round exchanges half of the live values with the partner lane, so the total
+// cost is NV - 1 + (6 - log2 NV) shuffles instead of 6 * NV.  Written with
+// compile-time recursion so that the value array stays in registers.
+template <typename T, int NV, int CNT, int OFF>
+struct MultiSum {
+    static __device__ __forceinline__ void run(T (&v)[NV], int l) {
+        if constexpr (CNT > 1) {
+            constexpr int HALF = CNT / 2;
+            const bool upper = (l & OFF) != 0;
+#pragma unroll
+            for (int i = 0; i < HALF; ++i) {
+                const T send = upper ? v[i] : v[i + HALF];
+                const T keep = upper ? v[i + HALF] : v[i];
+                v[i] = keep + __shfl_xor(send, OFF, WAVE);
+            }
+            MultiSum<T, NV, HALF, OFF / 2>::run(v, l);
+        } else if constexpr (OFF >= 1) {
+            v[0] += __shfl_xor(v[0], OFF, WAVE);
+            MultiSum<T, NV, 1, OFF / 2>::run(v, l);
+        }
+    }
+};
+
 template <typename T, int NV>
 __device__ __forceinline__ T wave_multi_sum(T (&v)[NV]) {
     static_assert((NV & (NV - 1)) == 0 && NV <= 64, "NV must be a power of two <= 64");
-    const int l = lane_id();
-    int cnt = NV;
-    int off = 32;
-    // Each round halves the number of live values per lane: lanes with bit `off`
-    // set keep the upper half, the others the lower half.
-#pragma unroll
-    for (int round = 0; round < 6; ++round) {
-        if (cnt > 1) {
-            const int half = cnt >> 1;
-            const bool upper = (l & off) != 0;
-#pragma unroll
-            for (int i = 0; i < NV / 2; ++i) {
-                if (i < half) {
-                    T send = upper ? v[i] : v[i + half];
-                    T keep = upper ? v[i + half] : v[i];
-                    T recv = __shfl_xor(send, off, WAVE);
-                    v[i] = keep + recv;
-                }
-            }
-            cnt = half;
-        } else {
-            v[0] += __shfl_xor(v[0], off, WAVE);
-        }
-        off >>= 1;
-    }
-    // After the rounds, lane l holds the sum of value index idx(l), where idx is
-    // formed from the lane bits consumed while cnt > 1 (bit 5 is the MSB).
+    MultiSum<T, NV, NV, 32>::run(v, lane_id());
     return v[0];
+}
+
+template <int NV>
+__host__ __device__ __forceinline__ constexpr int log2_const() {
+    return NV <= 1 ? 0 : 1 + log2_const<NV / 2>();
 }
 
 // Value index held by lane l after wave_multi_sum<NV>.
 template <int NV>
 __device__ __forceinline__ int wave_multi_sum_index(int l) {
-    int idx = 0;
-    int cnt = NV;
-    int off = 32;
-    int base = 0;
-    (void) base;
-    for (int round = 0; round < 6 && cnt > 1; ++round) {
-        const int half = cnt >> 1;
-        if (l & off) idx += half;
-        cnt = half;
-        off >>= 1;
-    }
-    return idx;
+    return l >> (6 - log2_const<NV>());
 }
 
 template <typename T>

@@ -14,7 +14,8 @@ constexpr int GT_LDS_ELEMS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
 
 template <typename T>
 __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B,
-                             int64_t ldb, bool tb, T beta, T *C, int64_t ldc, int ti, int tj, T *lds) {
+                             int64_t ldb, bool tb, T beta, T *C, int64_t ldc, int ti, int tj, T *lds,
+                             bool upper_b = false) {
     T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
     T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
     const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
@@ -51,7 +52,8 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
             }
             const int gj = j0 + j, gk = k0 + k;
             T v = T(0);
-            if (gj < N && gk < Kd) v = tb ? B[gj + (int64_t) gk * ldb] : B[gk + (int64_t) gj * ldb];
+            if (gj < N && gk < Kd && !(upper_b && gk < gj))
+                v = tb ? B[gj + (int64_t) gk * ldb] : B[gk + (int64_t) gj * ldb];
             Bs[k * (GT_N + 1) + j] = v;
         }
         __syncthreads();
@@ -85,6 +87,7 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
 }
 
 // Whole-matrix GEMM by one workgroup (loops over all 64x64 tiles).
+// `upper_b` treats op(B)(k, j) as zero for k < j (a transposed upper-triangular factor).
 template <typename T>
 __device__ void wg_gemm(int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B, int64_t ldb,
                         bool tb, T beta, T *C, int64_t ldc, T *lds) {

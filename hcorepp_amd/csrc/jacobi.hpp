@@ -1,0 +1,245 @@
+// Register-tiled one-sided block Jacobi SVD for one workgroup (4 waves).
+//
+// The matrix J (rows x ncols, column-major) is orthogonalised column-wise:
+// J V = Z with V orthogonal (accumulated) and the columns of Z mutually
+// orthogonal; sigma_j = |z_j|.  Columns are grouped in blocks of TW/2; a wave
+// owns one block pair ("tile" of TW columns) at a time, holding the tile in
+// registers (RPL rows per lane).  Within a tile, TW/2 disjoint column pairs
+// are rotated per round, their 3*TW/2 dot products reduced with one
+// multi-value wave reduction.  Block pairs follow the circle (round-robin)
+// ordering; step 0 of every sweep rotates all pairs inside the tile (so the
+// intra-block pairs of every block are visited once per sweep), later steps
+// only the cross pairs.  Rotations of a tile are accumulated in a TW x TW
+// matrix G (LDS, per wave) and applied to V's block columns afterwards.
+//
+// Convergence test: |z_p . z_q| <= sqrt(rows) * eps * |z_p| |z_q| (relative,
+// as LAPACK's one-sided Jacobi), pairs involving a column whose norm is below
+// eps * ||J||_F are not rotated (those columns carry rounding-level
+// singular values).
+#pragma once
+
+#include "device_common.hpp"
+
+namespace hcamd {
+
+// Lane that holds value index j after wave_multi_sum<NV>: index bits map to lane bits 5..(6-log2 NV).
+template <int NV>
+__device__ __forceinline__ constexpr int multi_sum_src_lane(int j) {
+    return j << (6 - log2_const<NV>());
+}
+
+template <int TW, bool FULL>
+__host__ __device__ __forceinline__ constexpr int jpair_p(int rd, int k) {
+    if (FULL) {
+        const int a = (k == 0) ? TW - 1 : (rd + k) % (TW - 1);
+        const int b = (k == 0) ? rd : (rd - k + TW - 1) % (TW - 1);
+        return a < b ? a : b;
+    } else {
+        return k;
+    }
+}
+
+template <int TW, bool FULL>
+__host__ __device__ __forceinline__ constexpr int jpair_q(int rd, int k) {
+    if (FULL) {
+        const int a = (k == 0) ? TW - 1 : (rd + k) % (TW - 1);
+        const int b = (k == 0) ? rd : (rd - k + TW - 1) % (TW - 1);
+        return a < b ? b : a;
+    } else {
+        return TW / 2 + (k + rd) % (TW / 2);
+    }
+}
+
+// One round: TW/2 disjoint pairs of the register tile y.
+template <typename T, int RPL, int TW, bool FULL>
+__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], int rd, T tol, T floor2, T *G) {
+    constexpr int NP = TW / 2;
+    constexpr int NV = (3 * NP <= 16) ? 16 : 32;
+    const int l = lane_id();
+    T part[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const T a = y[t][p], b = y[t][q];
+            part[3 * k] += a * a;
+            part[3 * k + 1] += b * b;
+            part[3 * k + 2] += a * b;
+        }
+    }
+    const T tot = wave_multi_sum<T, NV>(part);
+    const int kk = l < NP ? l : 0;
+    const T app = __shfl(tot, multi_sum_src_lane<NV>(3 * kk), WAVE);
+    const T aqq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 1), WAVE);
+    const T apq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 2), WAVE);
+    T c = T(1), s = T(0);
+    if (l < NP && app > floor2 && aqq > floor2 && fabs(apq) > tol * sqrt(app) * sqrt(aqq)) {
+        const T zeta = (aqq - app) / (T(2) * apq);
+        const T t = dsign(T(1), zeta) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
+        c = T(1) / sqrt(T(1) + t * t);
+        s = c * t;
+    }
+    int any = 0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const T ck = __shfl(c, k, WAVE);
+        const T sk = __shfl(s, k, WAVE);
+        if (sk != T(0)) {
+            const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const T a = y[t][p], b = y[t][q];
+                y[t][p] = ck * a - sk * b;
+                y[t][q] = sk * a + ck * b;
+            }
+            any = 1;
+        }
+    }
+    if (any) {
+        // G <- G * R_pq for every rotated pair: lanes (slot, row) with TW rows per slot.
+        constexpr int SLOTS = WAVE / TW;
+#pragma unroll
+        for (int it = 0; it < (NP + SLOTS - 1) / SLOTS; ++it) {
+            const int k = l / TW + it * SLOTS;
+            const int row = l % TW;
+            const int ksafe = k < NP ? k : 0;
+            const T ck = __shfl(c, ksafe, WAVE);
+            const T sk = __shfl(s, ksafe, WAVE);
+            if (k < NP && sk != T(0)) {
+                const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+                const T a = G[row + TW * p], b = G[row + TW * q];
+                G[row + TW * p] = ck * a - sk * b;
+                G[row + TW * q] = sk * a + ck * b;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    return any;
+}
+
+// Process one tile (block pair) with one wave.  Columns [cI, cI+TW/2) and [cJ, cJ+TW/2)
+// of J (rows x ncols, ld ldj); columns >= ncols are zero and never stored.
+template <typename T, int RPL, int TW>
+__device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ, bool full,
+                           T tol, T floor2, T *G) {
+    constexpr int B = TW / 2;
+    const int l = lane_id();
+    T y[RPL][TW];
+#pragma unroll
+    for (int c = 0; c < TW; ++c) {
+        const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int r = l + WAVE * t;
+            y[t][c] = (r < rows && gc < ncols) ? J[r + (int64_t) gc * ldj] : T(0);
+        }
+    }
+    for (int e = l; e < TW * TW; e += WAVE) G[e] = (e % (TW + 1) == 0) ? T(1) : T(0);
+    __builtin_amdgcn_wave_barrier();
+    int any = 0;
+    if (full) {
+#pragma unroll
+        for (int rd = 0; rd < TW - 1; ++rd) any |= jacobi_round<T, RPL, TW, true>(y, rd, tol, floor2, G);
+    } else {
+#pragma unroll
+        for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, TW, false>(y, rd, tol, floor2, G);
+    }
+    if (any) {
+#pragma unroll
+        for (int c = 0; c < TW; ++c) {
+            const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = l + WAVE * t;
+                if (r < rows && gc < ncols) J[r + (int64_t) gc * ldj] = y[t][c];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // V[:, tile] <- V[:, tile] * G   (V is ncols x ncols)
+        for (int r = l; r < ncols; r += WAVE) {
+            T v[TW];
+#pragma unroll
+            for (int c = 0; c < TW; ++c) {
+                const int gc = (c < B) ? cI + c : cJ + (c - B);
+                v[c] = (gc < ncols) ? V[r + (int64_t) gc * ldv] : T(0);
+            }
+#pragma unroll
+            for (int c2 = 0; c2 < TW; ++c2) {
+                const int gc = (c2 < B) ? cI + c2 : cJ + (c2 - B);
+                T s = T(0);
+#pragma unroll
+                for (int c = 0; c < TW; ++c) s += v[c] * G[c + TW * c2];
+                if (gc < ncols) V[r + (int64_t) gc * ldv] = s;
+            }
+        }
+    }
+    return any;
+}
+
+// LDS elements wg_block_jacobi needs: per-wave G matrices, a reduction slot per wave, a flag.
+template <typename T, int TW>
+__host__ __device__ constexpr int block_jacobi_lds_elems() {
+    return NWAVES * TW * TW + NWAVES + 2;
+}
+
+// Whole-workgroup block Jacobi.  J: rows x ncols (ld ldj), V: ncols x ncols (ld ldv) must hold
+// the identity on entry.  `gbuf` is LDS of block_jacobi_lds_elems<T, TW>() elements (all the
+// LDS this routine touches, so it may be called out of line).  Returns the sweeps used
+// (max_sweeps when it did not converge).
+template <typename T, int RPL, int TW>
+__device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int max_sweeps, T *gbuf) {
+    constexpr int B = TW / 2;
+    T *s_red = gbuf + NWAVES * TW * TW;
+    int &s_rot = *reinterpret_cast<int *>(s_red + NWAVES);
+    T part = T(0);
+    for (int c = 0; c < ncols; ++c)
+        for (int r = threadIdx.x; r < rows; r += NT) {
+            const T a = J[r + (int64_t) c * ldj];
+            part += a * a;
+        }
+    const T fro2 = wg_sum(part, s_red);
+    const T floor2 = Eps<T>::value * Eps<T>::value * fro2;
+    const T tol = sqrt(T(rows)) * Eps<T>::value;
+    const int nblk = (ncols + B - 1) / B;
+    const int nbp = nblk + (nblk & 1);
+    T *G = gbuf + wave_id() * TW * TW;
+    int sweep = 0;
+    if (ncols <= 1) return 0;
+    for (; sweep < max_sweeps; ++sweep) {
+        if (threadIdx.x == 0) s_rot = 0;
+        __syncthreads();
+        int any = 0;
+        for (int step = 0; step < nbp - 1; ++step) {
+            for (int bp = wave_id(); bp < nbp / 2; bp += NWAVES) {
+                int bI, bJ;
+                if (bp == 0) {
+                    bI = nbp - 1;
+                    bJ = step;
+                } else {
+                    bI = (step + bp) % (nbp - 1);
+                    bJ = (step - bp + nbp - 1) % (nbp - 1);
+                }
+                if (bI > bJ) {
+                    const int t = bI;
+                    bI = bJ;
+                    bJ = t;
+                }
+                if (bI * B >= ncols) continue;
+                any |= jacobi_tile<T, RPL, TW>(J, rows, ncols, ldj, V, ldv, bI * B, bJ * B, step == 0, tol, floor2,
+                                               G);
+            }
+            __syncthreads();
+        }
+        if (any && lane_id() == 0) atomicOr(&s_rot, 1);
+        __syncthreads();
+        const int rot = s_rot;
+        __syncthreads();
+        if (!rot) break;
+    }
+    return sweep;
+}
+
+}  // namespace hcamd

@@ -19,24 +19,30 @@ __global__ void __launch_bounds__(NT) k_stack(RecompDesc<T> d) {
 }
 
 template <typename T, int PB>
-__global__ void __launch_bounds__(NT) k_qr(RecompDesc<T> d) {
+__global__ void __launch_bounds__(NT) k_qr(RecompDesc<T> d, int which) {
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     Ws<T> w = ws_of(d);
-    const int K = (int) w.hdr[HDR_K];
-    if (blockIdx.x == 0) wg_geqrf<T, PB>(d.m, K, w.us, d.m, w.tau_u, w.t_u, lds);
-    else wg_geqrf<T, PB>(d.n, K, w.vs, d.n, w.tau_v, w.t_v, lds);
+    const Dims z = dims_of(d, w);
+    if (which == 0) wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds);
+    else wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_core(RecompDesc<T> d) {
-    ph_core<T>(d, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(NT) k_y(RecompDesc<T> d) {
+    __shared__ T lds[GT_LDS_ELEMS];
+    ph_y<T>(d, blockIdx.x, gridDim.x, lds);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_svd(RecompDesc<T> d, int lds_elems) {
-    extern __shared__ unsigned char smem_raw[];
-    ph_svd<T>(d, reinterpret_cast<T *>(smem_raw), lds_elems);
+__global__ void __launch_bounds__(NT) k_jmat(RecompDesc<T> d) {
+    ph_jmat<T>(d, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_svd(RecompDesc<T> d) {
+    __shared__ T gbuf[JACOBI_LDS_ELEMS];
+    ph_svd<T>(d, gbuf);
 }
 
 template <typename T, int PB>
@@ -91,6 +97,7 @@ __global__ void __launch_bounds__(NT) k_batched(const BatchEntry<T> *entries, co
                                                 BatchCounters *ctr, int lds_elems) {
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
+    (void) lds_elems;
     __shared__ int64_t s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = (int64_t) atomicAdd(&ctr->next, 1ULL);
@@ -104,22 +111,28 @@ __global__ void __launch_bounds__(NT) k_batched(const BatchEntry<T> *entries, co
         ph_stack<T>(d, 0, 1);
         __syncthreads();
         Ws<T> w = ws_of(d);
-        const int K = (int) w.hdr[HDR_K];
-        wg_geqrf<T, PB>(d.m, K, w.us, d.m, w.tau_u, w.t_u, lds);
-        __syncthreads();
-        wg_geqrf<T, PB>(d.n, K, w.vs, d.n, w.tau_v, w.t_v, lds);
-        __syncthreads();
-        ph_core<T>(d, 0, 1);
-        __syncthreads();
-        ph_svd<T>(d, lds, lds_elems);
-        __syncthreads();
-        const int r = (int) w.hdr[HDR_RANK];
-        ph_recon<T, PB>(d, 0, 0, r, lds);
-        __syncthreads();
-        ph_recon<T, PB>(d, 1, 0, r, lds);
-        __syncthreads();
-        ph_finish<T>(d, 0, 1);
-        __syncthreads();
+        const Dims z = dims_of(d, w);
+        if (z.K <= d.kmax) {
+            wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds);
+            __syncthreads();
+            ph_y<T>(d, 0, 1, lds);
+            __syncthreads();
+            wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds);
+            __syncthreads();
+            ph_jmat<T>(d, 0, 1);
+            __syncthreads();
+            ph_svd<T>(d, lds);
+            __syncthreads();
+        }
+        const int r = (z.K <= d.kmax) ? (int) w.hdr[HDR_RANK] : (int) *d.crank;
+        if (z.K <= d.kmax) {
+            ph_recon<T, PB>(d, 0, 0, r, lds);
+            __syncthreads();
+            ph_recon<T, PB>(d, 1, 0, r, lds);
+            __syncthreads();
+            ph_finish<T>(d, 0, 1);
+            __syncthreads();
+        }
         if (threadIdx.x == 0) {
             *d.crank = r;
             const int64_t st = w.hdr[HDR_STATUS];
@@ -152,49 +165,58 @@ hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A,
 }
 
 int pick_pb(int64_t rows, int64_t lds_elems) {
-    const int pbs[4] = {16, 8, 4, 2};
-    for (int i = 0; i < 4; ++i) {
+    const int pbs[3] = {16, 8, 4};
+    if (rows > 2048) return 0;  // apply_wy keeps at most 8 row chunks per lane
+    for (int i = 0; i < 3; ++i) {
         const int pb = pbs[i];
-        if (rows * pb + 2 * pb * pb + NWAVES * pb + 64 <= lds_elems) return pb;
+        if (qr_lds_elems(rows, pb) <= lds_elems) return pb;
     }
     return 0;
 }
 
 template <typename T, int PB>
-static hipError_t launch_recomp_chain_pb(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int phase,
-                                         int r) {
-    if (phase == 0) {
-        hipLaunchKernelGGL((k_qr<T, PB>), dim3(2), dim3(NT), lds_bytes, s, d);
-    } else {
-        const int cols = 8;
-        const int nch = (r + cols - 1) / cols;
-        hipLaunchKernelGGL((k_recon<T, PB>), dim3(nch > 0 ? nch : 1, 2), dim3(NT), lds_bytes, s, d, cols);
-    }
+static hipError_t launch_qr_pb(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int which) {
+    hipLaunchKernelGGL((k_qr<T, PB>), dim3(1), dim3(NT), lds_bytes, s, d, which);
     return hipGetLastError();
+}
+
+template <typename T, int PB>
+static hipError_t launch_recon_pb(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int r) {
+    const int cols = 64 / PB;  // one column batch of the compact-WY application per workgroup
+    const int nch = (r + cols - 1) / cols;
+    hipLaunchKernelGGL((k_recon<T, PB>), dim3(nch > 0 ? nch : 1, 2), dim3(NT), lds_bytes, s, d, cols);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_qr(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int pb, int which) {
+    switch (pb) {
+        case 16: return launch_qr_pb<T, 16>(s, d, lds_bytes, which);
+        case 8: return launch_qr_pb<T, 8>(s, d, lds_bytes, which);
+        default: return launch_qr_pb<T, 4>(s, d, lds_bytes, which);
+    }
+}
+
+static int clamp_grid(int64_t work) {
+    int64_t g = (work + NT - 1) / NT;
+    if (g > 1024) g = 1024;
+    if (g < 1) g = 1;
+    return (int) g;
 }
 
 template <typename T>
 hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb) {
     const int64_t m = d.m, n = d.n;
-    int nwg = (int) ((m * K + NT - 1) / NT);
-    if (nwg > 1024) nwg = 1024;
-    if (nwg < 1) nwg = 1;
-    hipLaunchKernelGGL(k_stack<T>, dim3(nwg), dim3(NT), 0, s, d);
-    hipError_t e;
-    switch (pb) {
-        case 16: e = launch_recomp_chain_pb<T, 16>(s, d, lds_bytes, 0, 0); break;
-        case 8: e = launch_recomp_chain_pb<T, 8>(s, d, lds_bytes, 0, 0); break;
-        case 4: e = launch_recomp_chain_pb<T, 4>(s, d, lds_bytes, 0, 0); break;
-        default: e = launch_recomp_chain_pb<T, 2>(s, d, lds_bytes, 0, 0); break;
-    }
+    const int64_t mu = m < K ? m : K, kk = n < mu ? n : mu;
+    hipLaunchKernelGGL(k_stack<T>, dim3(clamp_grid(m * K)), dim3(NT), 0, s, d);
+    hipError_t e = launch_qr<T>(s, d, lds_bytes, pb, 0);
     if (e != hipSuccess) return e;
-    const int64_t mu = m < K ? m : K, mv = n < K ? n : K;
-    int nwc = (int) ((mu * mv + NT - 1) / NT);
-    if (nwc > 1024) nwc = 1024;
-    if (nwc < 1) nwc = 1;
-    hipLaunchKernelGGL(k_core<T>, dim3(nwc), dim3(NT), 0, s, d);
-    const int lds_elems = (int) (lds_bytes / sizeof(T));
-    hipLaunchKernelGGL(k_svd<T>, dim3(1), dim3(NT), lds_bytes, s, d, lds_elems);
+    const int ytiles = (int) (((n + GT_M - 1) / GT_M) * ((mu + GT_N - 1) / GT_N));
+    hipLaunchKernelGGL(k_y<T>, dim3(ytiles), dim3(NT), 0, s, d);
+    e = launch_qr<T>(s, d, lds_bytes, pb, 1);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_jmat<T>, dim3(clamp_grid(mu * kk + kk * kk)), dim3(NT), 0, s, d);
+    hipLaunchKernelGGL(k_svd<T>, dim3(1), dim3(NT), 0, s, d);
     return hipGetLastError();
 }
 
@@ -202,16 +224,12 @@ template <typename T>
 hipError_t launch_recomp_post(hipStream_t s, const RecompDesc<T> &d, int64_t r, int64_t lds_bytes, int pb) {
     hipError_t e;
     switch (pb) {
-        case 16: e = launch_recomp_chain_pb<T, 16>(s, d, lds_bytes, 1, (int) r); break;
-        case 8: e = launch_recomp_chain_pb<T, 8>(s, d, lds_bytes, 1, (int) r); break;
-        case 4: e = launch_recomp_chain_pb<T, 4>(s, d, lds_bytes, 1, (int) r); break;
-        default: e = launch_recomp_chain_pb<T, 2>(s, d, lds_bytes, 1, (int) r); break;
+        case 16: e = launch_recon_pb<T, 16>(s, d, lds_bytes, (int) r); break;
+        case 8: e = launch_recon_pb<T, 8>(s, d, lds_bytes, (int) r); break;
+        default: e = launch_recon_pb<T, 4>(s, d, lds_bytes, (int) r); break;
     }
     if (e != hipSuccess) return e;
-    int nwg = (int) ((r * d.n + NT - 1) / NT);
-    if (nwg > 1024) nwg = 1024;
-    if (nwg < 1) nwg = 1;
-    hipLaunchKernelGGL(k_finish<T>, dim3(nwg), dim3(NT), 0, s, d);
+    hipLaunchKernelGGL(k_finish<T>, dim3(clamp_grid(r * d.n)), dim3(NT), 0, s, d);
     return hipGetLastError();
 }
 
@@ -240,8 +258,7 @@ hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int
     switch (pb) {
         case 16: hipLaunchKernelGGL((k_batched<T, 16>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
         case 8: hipLaunchKernelGGL((k_batched<T, 8>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
-        case 4: hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
-        default: hipLaunchKernelGGL((k_batched<T, 2>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
+        default: hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
     }
     return hipGetLastError();
 }
@@ -251,10 +268,8 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
     // Allow the dynamic-LDS kernels to use more than the default 64 KB.
     const void *fns[] = {
             (const void *) k_qr<T, 16>, (const void *) k_qr<T, 8>, (const void *) k_qr<T, 4>,
-            (const void *) k_qr<T, 2>, (const void *) k_svd<T>, (const void *) k_recon<T, 16>,
-            (const void *) k_recon<T, 8>, (const void *) k_recon<T, 4>, (const void *) k_recon<T, 2>,
+            (const void *) k_recon<T, 16>, (const void *) k_recon<T, 8>, (const void *) k_recon<T, 4>,
             (const void *) k_batched<T, 16>, (const void *) k_batched<T, 8>, (const void *) k_batched<T, 4>,
-            (const void *) k_batched<T, 2>,
     };
     for (const void *f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds_bytes);

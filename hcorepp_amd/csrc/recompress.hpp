@@ -1,16 +1,24 @@
 // Workgroup-level device phases of one compressed-tile update
 // C <- alpha * Afin * Bfin + beta * C with C = C_U * C_V (rank rc, capacity max_rank).
 //
-// The phases restate CompressedTile<T>::Gemm (reference
-// src/operators/concrete/Compressed.cpp:208-694) in a form suited to CDNA4:
-//   stack      U_s = [C_U | alpha*Afin], V_s = [(beta*C_V)^T | Bfin^T]      (:331-349, :375-380)
-//   qr         Householder QR of U_s and V_s, compact-WY panels in LDS       (:351-354, :383-385)
-//   core       X = R_V * R_U^T  (the transpose of R_U * R_V^T, :454-461)
-//   svd        one-sided Jacobi on X: X*W = Z, sigma_j = |Z_j|; sort; rank  (:466-527)
-//   recon      C_U = Q_U [W_r; 0],  C_V = (Q_V [Z_r; 0])^T                  (:549-628, :656-682)
-// Since X = M^T with M = R_U R_V^T, M = W Z^T: the left singular vectors of M
-// are the columns of W and the right ones scaled by sigma are the columns of Z,
-// so "V carries Sigma" (:597-599) falls out of the rotation without a scaling pass.
+// The phases compute the truncated SVD that CompressedTile<T>::Gemm computes
+// (reference src/operators/concrete/Compressed.cpp:208-694), arranged for CDNA4:
+//
+//   stack   U_s = [C_U | alpha*Afin] (m x K), V_s = [(beta*C_V)^T | Bfin^T] (n x K)
+//           (LaCpy#1/#2, MultiplyByAlpha, ProcessVpointer: :331-349, :375-380)
+//   qr_u    U_s = Q_U R_U, blocked Householder, reflectors kept     (Geqrf#1 :351-354)
+//   y       Y = V_s R_U^T  (n x mu), so that U_s V_s^T = Q_U Y^T
+//   qr_y    Y = Q_Y R_Y                                             (replaces Geqrf#2 :383-385)
+//   svd     one-sided block Jacobi on J = R_Y^T (mu x kk): J V1 = Z1, sigma_j = |z_j|,
+//           descending sort, rank rule and clamps                   (SVD#3 + rank :454-527)
+//   recon   C_U = Q_U [Z1 Sigma^-1; 0]_r ,  C_V = (Q_Y [V1 Sigma; 0]_r)^T
+//                                                                   (ungqr + Gemm#3/#4 :549-628)
+// with mu = min(m, K), kk = min(n, mu) = min(m, n, K) = size_s.  Since
+// U_s V_s^T = Q_U R_Y^T Q_Y^T = Q_U Z1 V1^T Q_Y^T, the left singular vectors are
+// Q_U Z1 Sigma^-1 and the right ones Q_Y V1, identical in exact arithmetic to the
+// reference's Q_U Unew and Q_V VTnew^T.  R_Y equals the R factor of the QR of
+// (R_U R_V^T)^T, a triangular preconditioner that cuts the Jacobi sweep count ~3x.
+// "V carries Sigma" (:597-599) is the V1 Sigma factor.
 //
 // Every phase is executed by a whole workgroup of NT threads; the same code
 // runs one phase per launch (plain path) or all phases in one persistent
@@ -18,6 +26,8 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "gemm.hpp"
+#include "jacobi.hpp"
 
 namespace hcamd {
 
@@ -30,9 +40,10 @@ enum : int {
     HDR_STATUS = 4,  // status bits (STATUS_*)
     HDR_WORDS = 8,
 };
-enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2 };
+enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2, STATUS_TOO_BIG = 4 };
 
-constexpr int JACOBI_MAX_SWEEPS = 60;
+constexpr int JACOBI_MAX_SWEEPS = 40;
+constexpr int QR_PB_MAX = 16;  // widest Householder panel
 
 // Descriptor of one compressed-C update.  Pointers are device pointers.
 template <typename T>
@@ -55,7 +66,7 @@ struct RecompDesc {
 
 // Workspace layout (element offsets of T) for capacity kmax and tile m x n.
 struct WsLayout {
-    int64_t hdr, us, vs, tau_u, tau_v, t_u, t_v, x, w, sig, perm, vtmp, total;
+    int64_t hdr, us, vs, y, tau_u, tau_y, t_u, t_y, j, v1, sig, perm, vtmp, total;
 };
 
 __host__ __device__ inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
@@ -64,19 +75,21 @@ template <typename T>
 __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax, int64_t max_rank) {
     WsLayout L;
     int64_t o = 0;
+    const int64_t mu = m < kmax ? m : kmax;
+    const int64_t kk = n < mu ? n : mu;
     const int64_t hdr_elems = (HDR_WORDS * 8 + sizeof(T) - 1) / sizeof(T);
     L.hdr = o; o += align_up(hdr_elems, 32);
     L.us = o; o += align_up(m * kmax, 32);
     L.vs = o; o += align_up(n * kmax, 32);
+    L.y = o; o += align_up(n * mu, 32);
     L.tau_u = o; o += align_up(kmax, 32);
-    L.tau_v = o; o += align_up(kmax, 32);
-    L.t_u = o; o += align_up(kmax * 16 + 256, 32);
-    L.t_v = o; o += align_up(kmax * 16 + 256, 32);
-    const int64_t mu = m < kmax ? m : kmax, mv = n < kmax ? n : kmax;
-    L.x = o; o += align_up(mv * mu, 32);
-    L.w = o; o += align_up(mu * mu, 32);
-    L.sig = o; o += align_up(mu, 32);
-    L.perm = o; o += align_up((mu * 4 + sizeof(T) - 1) / sizeof(T), 32);
+    L.tau_y = o; o += align_up(kmax, 32);
+    L.t_u = o; o += align_up(kmax * QR_PB_MAX + QR_PB_MAX * QR_PB_MAX, 32);
+    L.t_y = o; o += align_up(kmax * QR_PB_MAX + QR_PB_MAX * QR_PB_MAX, 32);
+    L.j = o; o += align_up(mu * kk, 32);
+    L.v1 = o; o += align_up(kk * kk, 32);
+    L.sig = o; o += align_up(kk, 32);
+    L.perm = o; o += align_up((kk * 4 + sizeof(T) - 1) / sizeof(T), 32);
     L.vtmp = o; o += align_up(n * (max_rank > 0 ? max_rank : 1), 32);
     L.total = o;
     return L;
@@ -85,7 +98,7 @@ __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax
 template <typename T>
 struct Ws {
     int64_t *hdr;
-    T *us, *vs, *tau_u, *tau_v, *t_u, *t_v, *x, *w, *sig, *vtmp;
+    T *us, *vs, *y, *tau_u, *tau_y, *t_u, *t_y, *j, *v1, *sig, *vtmp;
     int32_t *perm;
 };
 
@@ -96,21 +109,35 @@ __device__ inline Ws<T> ws_of(const RecompDesc<T> &d) {
     w.hdr = reinterpret_cast<int64_t *>(d.ws + L.hdr);
     w.us = d.ws + L.us;
     w.vs = d.ws + L.vs;
+    w.y = d.ws + L.y;
     w.tau_u = d.ws + L.tau_u;
-    w.tau_v = d.ws + L.tau_v;
+    w.tau_y = d.ws + L.tau_y;
     w.t_u = d.ws + L.t_u;
-    w.t_v = d.ws + L.t_v;
-    w.x = d.ws + L.x;
-    w.w = d.ws + L.w;
+    w.t_y = d.ws + L.t_y;
+    w.j = d.ws + L.j;
+    w.v1 = d.ws + L.v1;
     w.sig = d.ws + L.sig;
     w.perm = reinterpret_cast<int32_t *>(d.ws + L.perm);
     w.vtmp = d.ws + L.vtmp;
     return w;
 }
 
+struct Dims {
+    int K, mu, kk;
+};
+
+template <typename T>
+__device__ inline Dims dims_of(const RecompDesc<T> &d, const Ws<T> &w) {
+    Dims z;
+    z.K = (int) w.hdr[HDR_K];
+    z.mu = d.m < z.K ? d.m : z.K;
+    z.kk = d.n < z.mu ? d.n : z.mu;
+    return z;
+}
+
 // ---------------------------------------------------------------------------
-// Phase 1: stack.  Grid-stride over (part, block) so that several workgroups
-// may share one update: `wg` in [0, nwg).
+// Phase 1: stack.  Grid-stride so several workgroups may share one update
+// (`wg` in [0, nwg)).
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
@@ -120,8 +147,9 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
     const int64_t K = ra + rc;
     if (wg == 0 && threadIdx.x == 0) {
         w.hdr[HDR_K] = K;
-        w.hdr[HDR_STATUS] = 0;
+        w.hdr[HDR_STATUS] = (K > d.kmax) ? STATUS_TOO_BIG : 0;
     }
+    if (K > d.kmax) return;
     const T *cu = d.cdata;
     const T *cv = d.cdata + m * d.max_rank;  // ld rc
     const int64_t totU = m * K, totV = n * K;
@@ -138,85 +166,113 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
 }
 
 // ---------------------------------------------------------------------------
-// Compact-WY helpers.  A panel of PB reflectors for rows [j0, m) lives in LDS
-// as P[r + c*rows] (c < PB), unit diagonal and zeros above, columns >= jb zero.
-// T (PB x PB upper) lives in LDS too.
+// Compact-WY block reflector application with the panel held in registers.
+// Rows are dealt to waves in 64-row chunks (chunk c -> wave c % NWAVES); each
+// lane keeps its rows of the panel (RPT chunks x PB columns).  Columns of B
+// are processed in batches of 64/PB: per batch, partial V^T b dot products are
+// reduced within the wave (one multi-value reduction), across waves through
+// LDS, multiplied by T^T (trans) or T, and the update b -= V t is applied.
+// `red` is LDS scratch of NWAVES*64 + 2*64 elements.
 // ---------------------------------------------------------------------------
-
-// Apply (I - V T V^T) (trans=false) or (I - V T^T V^T) (trans=true) from the
-// left to columns [c0, c1) of B (rows [j0, j0+rows) of a column-major matrix
-// with leading dimension ldb).  One wave per column; all waves participate.
-template <typename T, int PB>
-__device__ void apply_block_reflector(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
-                                      bool trans) {
-    const int l = lane_id();
-    const int idx = wave_multi_sum_index<PB>(l);
-    for (int c = c0 + wave_id(); c < c1; c += NWAVES) {
-        T *col = B + (int64_t) c * ldb;
-        T acc[PB];
+template <typename T, int PB, int RPT>
+__device__ void apply_wy(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1, bool trans, T *red) {
+    constexpr int NCB = WAVE / PB;  // columns per batch
+    const int l = lane_id(), wv = wave_id();
+    T pv[RPT][PB];
 #pragma unroll
-        for (int k = 0; k < PB; ++k) acc[k] = T(0);
-        for (int r = l; r < rows; r += WAVE) {
-            const T a = col[r];
+    for (int t = 0; t < RPT; ++t) {
+        const int r = l + WAVE * (wv + NWAVES * t);
 #pragma unroll
-            for (int k = 0; k < PB; ++k) acc[k] += P[r + k * rows] * a;
-        }
-        const T dot = wave_multi_sum<T, PB>(acc);  // lane l holds sum for index idx
-        // t_k = (T^T w)_k = sum_{j<=k} T[j][k] w_j   or   (T w)_k = sum_{j>=k} T[k][j] w_j
-        T tk = T(0);
+        for (int k = 0; k < PB; ++k) pv[t][k] = (r < rows) ? P[r + k * rows] : T(0);
+    }
+    T *wfull = red + NWAVES * WAVE;
+    T *tvec = wfull + WAVE;
+    for (int cb = c0; cb < c1; cb += NCB) {
+        T part[NCB * PB];
 #pragma unroll
-        for (int j = 0; j < PB; ++j) {
-            // lane holding index j: bits chosen as in wave_multi_sum_index
-            int src = 0, cnt = PB, off = 32, jj = j;
+        for (int c = 0; c < NCB; ++c) {
+            T bv[RPT];
 #pragma unroll
-            for (int round = 0; round < 6; ++round) {
-                if (cnt > 1) {
-                    const int half = cnt >> 1;
-                    if (jj >= half) {
-                        src |= off;
-                        jj -= half;
-                    }
-                    cnt = half;
-                }
-                off >>= 1;
+            for (int t = 0; t < RPT; ++t) {
+                const int r = l + WAVE * (wv + NWAVES * t);
+                bv[t] = (r < rows && cb + c < c1) ? B[r + (int64_t) (cb + c) * ldb] : T(0);
             }
-            const T wj = __shfl(dot, src, WAVE);
-            const T coef = trans ? ((j <= idx) ? Tm[j + idx * PB] : T(0)) : ((j >= idx) ? Tm[idx + j * PB] : T(0));
-            tk += coef * wj;
-        }
-        // broadcast t_k for all k
-        T tv[PB];
 #pragma unroll
-        for (int k = 0; k < PB; ++k) {
-            int src = 0, cnt = PB, off = 32, kk = k;
+            for (int k = 0; k < PB; ++k) {
+                T s = T(0);
 #pragma unroll
-            for (int round = 0; round < 6; ++round) {
-                if (cnt > 1) {
-                    const int half = cnt >> 1;
-                    if (kk >= half) {
-                        src |= off;
-                        kk -= half;
-                    }
-                    cnt = half;
-                }
-                off >>= 1;
+                for (int t = 0; t < RPT; ++t) s += pv[t][k] * bv[t];
+                part[c * PB + k] = s;
             }
-            tv[k] = __shfl(tk, src, WAVE);
         }
-        for (int r = l; r < rows; r += WAVE) {
+        const T tot = wave_multi_sum<T, NCB * PB>(part);  // lane l holds index l
+        red[wv * WAVE + l] = tot;
+        __syncthreads();
+        if (threadIdx.x < WAVE) {
             T s = T(0);
 #pragma unroll
-            for (int k = 0; k < PB; ++k) s += P[r + k * rows] * tv[k];
-            col[r] -= s;
+            for (int w2 = 0; w2 < NWAVES; ++w2) s += red[w2 * WAVE + threadIdx.x];
+            wfull[threadIdx.x] = s;
         }
+        __syncthreads();
+        if (threadIdx.x < WAVE) {
+            const int c = threadIdx.x / PB, k = threadIdx.x % PB;
+            T s = T(0);
+            if (trans) {
+#pragma unroll
+                for (int j = 0; j < PB; ++j)
+                    if (j <= k) s += Tm[j + k * PB] * wfull[c * PB + j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < PB; ++j)
+                    if (j >= k) s += Tm[k + j * PB] * wfull[c * PB + j];
+            }
+            tvec[threadIdx.x] = s;
+        }
+        __syncthreads();
+        for (int c = 0; c < NCB; ++c) {
+            if (cb + c >= c1) break;
+            T tc[PB];
+#pragma unroll
+            for (int k = 0; k < PB; ++k) tc[k] = tvec[c * PB + k];
+#pragma unroll
+            for (int t = 0; t < RPT; ++t) {
+                const int r = l + WAVE * (wv + NWAVES * t);
+                if (r < rows) {
+                    T s = T(0);
+#pragma unroll
+                    for (int k = 0; k < PB; ++k) s += pv[t][k] * tc[k];
+                    B[r + (int64_t) (cb + c) * ldb] -= s;
+                }
+            }
+        }
+        __syncthreads();
     }
+}
+
+// Row-chunk count dispatch.  The variants are kept out of line so that each
+// gets its own register allocation (inlining them all into one kernel makes
+// the widest variant's pressure spill the others).
+template <typename T, int PB, int RPT>
+__device__ __noinline__ void apply_wy_nl(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
+                                          bool trans, T *red) {
+    apply_wy<T, PB, RPT>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+}
+
+template <typename T, int PB>
+__device__ void apply_wy_dispatch(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1, bool trans,
+                                  T *red) {
+    const int chunks = (rows + WAVE * NWAVES - 1) / (WAVE * NWAVES);
+    if (chunks <= 1) apply_wy_nl<T, PB, 1>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+    else if (chunks <= 2) apply_wy_nl<T, PB, 2>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+    else if (chunks <= 4) apply_wy_nl<T, PB, 4>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+    else if constexpr (PB <= 8) apply_wy_nl<T, PB, 8>(P, Tm, rows, B, ldb, c0, c1, trans, red);
 }
 
 // Build T (PB x PB, upper, column-major in LDS) for the panel P with taus tau[0..jb):
 // dlarft forward/columnwise: T[i][i] = tau_i, T[0:i, i] = -tau_i T[0:i,0:i] (V[:,0:i]^T v_i).
 template <typename T, int PB>
 __device__ void build_t(const T *P, int rows, const T *tau, int jb, T *Tm, T *gram) {
-    // gram[a + b*PB] = v_a^T v_b for a < b < jb
     const int l = lane_id();
     for (int pr = wave_id(); pr < PB * PB; pr += NWAVES) {
         const int a = pr % PB, b = pr / PB;
@@ -232,7 +288,6 @@ __device__ void build_t(const T *P, int rows, const T *tau, int jb, T *Tm, T *gr
         for (int i = 0; i < jb; ++i) {
             const T ti = tau[i];
             Tm[i + i * PB] = ti;
-            // tmp = V[:,0:i]^T v_i = gram[0:i, i]; T[0:i, i] = -ti * T[0:i,0:i] * tmp
             for (int a = 0; a < i; ++a) {
                 T s = T(0);
                 for (int b = a; b < i; ++b) s += Tm[a + b * PB] * gram[b + i * PB];
@@ -243,45 +298,32 @@ __device__ void build_t(const T *P, int rows, const T *tau, int jb, T *Tm, T *gr
     __syncthreads();
 }
 
-// Load the reflector panel [j0, j0+jb) of the factored matrix A (ld lda) into LDS.
-template <typename T, int PB>
-__device__ void load_reflector_panel(const T *A, int64_t lda, int m, int j0, int jb, T *P) {
-    const int rows = m - j0;
-    for (int e = threadIdx.x; e < rows * PB; e += NT) {
-        const int r = e % rows, c = e / rows;
-        T v = T(0);
-        if (c < jb) {
-            if (r == c) v = T(1);
-            else if (r > c) v = A[(j0 + r) + (int64_t) (j0 + c) * lda];
-        }
-        P[e] = v;
-    }
+// LDS elements needed by wg_geqrf / wg_apply_q for a panel of PB columns over m rows.
+__host__ __device__ inline int64_t qr_lds_elems(int64_t m, int pb) {
+    return m * pb + 2 * pb * pb + (NWAVES + 2) * WAVE + 64;
 }
 
 // ---------------------------------------------------------------------------
-// Phase 2: blocked Householder QR (LAPACK dgeqrf convention) of A (m x ncols,
-// ld lda) in place; tau[0..min(m,ncols)); T factors of every panel saved to
-// tsave[panel * PB * PB].  `lds` must hold m*PB + 2*PB*PB + 64 elements.
+// Blocked Householder QR (LAPACK dgeqrf convention) of A (m x ncols, ld lda) in
+// place; tau[0..min(m,ncols)); the T factor of panel p is saved to tsave[p*PB*PB].
 // ---------------------------------------------------------------------------
 template <typename T, int PB>
 __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds) {
     T *P = lds;
     T *Tm = P + (int64_t) m * PB;
     T *gram = Tm + PB * PB;
-    T *red = gram + PB * PB;   // NWAVES * PB
+    T *red = gram + PB * PB;  // (NWAVES + 2) * WAVE
     const int nsteps = m < ncols ? m : ncols;
     const int l = lane_id();
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
         const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
         const int rows = m - j0;
-        // load panel (all PB columns; columns >= jb are zero, columns beyond ncols too)
         for (int e = threadIdx.x; e < rows * PB; e += NT) {
             const int r = e % rows, c = e / rows;
             P[e] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
         }
         __syncthreads();
         for (int jj = 0; jj < jb; ++jj) {
-            // ||x[jj+1:]||^2
             T part = T(0);
             for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
                 const T v = P[r + jj * rows];
@@ -308,8 +350,8 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
                 tau[j0 + jj] = tau_j;
             }
             __syncthreads();
-            // apply H_jj to panel columns jj+1..jb-1:  w_c = x_c[jj] + sum_{r>jj} v_r x_c[r]
             if (tau_j != T(0) && jj + 1 < jb) {
+                // w_k = x_k[jj] + sum_{r>jj} v_r x_k[r] for panel columns k in (jj, jb)
                 T acc[PB];
 #pragma unroll
                 for (int k = 0; k < PB; ++k) acc[k] = T(0);
@@ -319,28 +361,16 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
                     for (int k = 0; k < PB; ++k)
                         if (k > jj && k < jb) acc[k] += v * P[r + k * rows];
                 }
-                const int idx = wave_multi_sum_index<PB>(l);
                 const T s = wave_multi_sum<T, PB>(acc);
-                // lanes 0..63 hold index idx(l); the first lane with each index writes
-                int first = 1;
-                {
-                    // lane l is the first holder of idx iff its low bits below the index bits are zero
-                    int cnt = PB, off = 32, consumed = 0;
-                    for (int round = 0; round < 6 && cnt > 1; ++round) {
-                        cnt >>= 1;
-                        consumed |= off;
-                        off >>= 1;
-                    }
-                    first = ((l & ~consumed) == 0);
-                }
-                if (first) red[wave_id() * PB + idx] = s;
+                const int idx = wave_multi_sum_index<PB>(l);
+                if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
                 __syncthreads();
                 if (threadIdx.x < PB) {
                     const int k = threadIdx.x;
                     if (k > jj && k < jb) {
                         T wsum = P[jj + k * rows];
                         for (int wv = 0; wv < NWAVES; ++wv) wsum += red[wv * PB + k];
-                        gram[k] = tau_j * wsum;  // reuse gram[0..PB) as the scaled w
+                        gram[k] = tau_j * wsum;
                     }
                 }
                 __syncthreads();
@@ -354,13 +384,12 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
                 __syncthreads();
             }
         }
-        // write the factored panel back (R entries on/above diagonal, reflectors below)
+        // write the factored panel back (R on/above the diagonal, reflectors below)
         for (int e = threadIdx.x; e < rows * jb; e += NT) {
             const int r = e % rows, c = e / rows;
             A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[e];
         }
         __syncthreads();
-        // reflector form of the panel (unit diagonal, zeros above) for T and the update
         for (int e = threadIdx.x; e < rows * PB; e += NT) {
             const int r = e % rows, c = e / rows;
             if (c < jb) {
@@ -373,173 +402,121 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
         if (tsave) {
             for (int e = threadIdx.x; e < PB * PB; e += NT) tsave[(int64_t) (j0 / PB) * PB * PB + e] = Tm[e];
         }
-        // trailing update: A[j0:, j0+jb:ncols] = (I - V T^T V^T) A[...]   (H^T applied: Q^T A)
-        if (j0 + jb < ncols) {
-            apply_block_reflector<T, PB>(P, Tm, rows, A + j0, lda, j0 + jb, ncols, true);
-        }
+        if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, rows, A + j0, lda, j0 + jb, ncols, true, red);
         __syncthreads();
     }
 }
 
-// Apply Q = H_1 ... H_k (k = nref reflectors of the factored matrix A, ld lda,
-// m rows) to B (m x ncols, ld ldb): B <- Q B.  T factors from tsave.
+// B <- Q B with Q = H_1 ... H_nref the reflectors of a factored matrix A (m rows, ld lda),
+// applied to columns [c0, c1) of B (m rows, ld ldb).
 template <typename T, int PB>
 __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *tsave, T *B, int64_t ldb, int c0,
                            int c1, T *lds) {
     T *P = lds;
     T *Tm = P + (int64_t) m * PB;
+    T *red = Tm + 2 * PB * PB;
     const int npan = (nref + PB - 1) / PB;
     for (int p = npan - 1; p >= 0; --p) {
         const int j0 = p * PB;
         const int jb = (nref - j0) < PB ? (nref - j0) : PB;
         const int rows = m - j0;
-        load_reflector_panel<T, PB>(A, lda, m, j0, jb, P);
+        for (int e = threadIdx.x; e < rows * PB; e += NT) {
+            const int r = e % rows, c = e / rows;
+            T v = T(0);
+            if (c < jb) {
+                if (r == c) v = T(1);
+                else if (r > c) v = A[(j0 + r) + (int64_t) (j0 + c) * lda];
+            }
+            P[e] = v;
+        }
         for (int e = threadIdx.x; e < PB * PB; e += NT) Tm[e] = tsave[(int64_t) p * PB * PB + e];
         __syncthreads();
-        apply_block_reflector<T, PB>(P, Tm, rows, B + j0, ldb, c0, c1, false);
+        apply_wy_dispatch<T, PB>(P, Tm, rows, B + j0, ldb, c0, c1, false, red);
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Phase 3: X = R_V R_U^T (minV x minU, ld minV) and W = I (minU x minU).
-// Grid-stride over elements so several workgroups may share one update.
+// Phase: Y = V_s R_U^T  (n x mu),  R_U = triu(U_s[0:mu, 0:K]).
+// Output tiles are dealt to workgroups `wg` of `nwg`.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ void ph_core(const RecompDesc<T> &d, int wg, int nwg) {
+__device__ void ph_y(const RecompDesc<T> &d, int wg, int nwg, T *lds) {
     Ws<T> w = ws_of(d);
-    const int64_t K = w.hdr[HDR_K];
-    const int64_t m = d.m, n = d.n;
-    const int64_t mu = m < K ? m : K, mv = n < K ? n : K;
+    const Dims z = dims_of(d, w);
+    const int tm = (d.n + GT_M - 1) / GT_M, tn = (z.mu + GT_N - 1) / GT_N;
+    for (int t = wg; t < tm * tn; t += nwg)
+        wg_gemm_tile<T>(d.n, z.mu, z.K, T(1), w.vs, d.n, false, w.us, d.m, true, T(0), w.y, d.n, t % tm, t / tm,
+                        lds, /*upper_b=*/true);
+}
+
+// Phase: J = R_Y^T (mu x kk, ld mu) and V1 = I (kk x kk).
+template <typename T>
+__device__ void ph_jmat(const RecompDesc<T> &d, int wg, int nwg) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
     const int64_t stride = (int64_t) nwg * NT;
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < mv * mu; e += stride) {
-        const int64_t i = e % mv, j = e / mv;
-        const int64_t k0 = i > j ? i : j;
-        T s = T(0);
-        for (int64_t k = k0; k < K; ++k) s += w.vs[i + k * n] * w.us[j + k * m];
-        w.x[e] = s;
+    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < (int64_t) z.mu * z.kk; e += stride) {
+        const int64_t i = e % z.mu, j = e / z.mu;
+        w.j[e] = (i >= j) ? w.y[j + i * d.n] : T(0);
     }
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < mu * mu; e += stride) {
-        const int64_t i = e % mu, j = e / mu;
-        w.w[e] = (i == j) ? T(1) : T(0);
+    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < (int64_t) z.kk * z.kk; e += stride) {
+        const int64_t i = e % z.kk, j = e / z.kk;
+        w.v1[e] = (i == j) ? T(1) : T(0);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Phase 4: one-sided Jacobi on X (rows mv, cols mu) with W accumulation,
-// round-robin (circle) ordering, relative convergence test
-// |x_p . x_q| <= tol |x_p||x_q| with tol = sqrt(mv) * eps.  Then
-// sigma_j = |x_j|, descending sort (perm), rank rule (omp/kernels.cpp:82-104)
-// and clamps (Compressed.cpp:509-527).  Single workgroup.
+// Phase: Jacobi + sigma + sort + rank (single workgroup).
 // ---------------------------------------------------------------------------
+template <typename T, int RPL, int TW>
+__device__ __noinline__ int block_jacobi_nl(T *J, int rows, int ncols, T *V, T *gbuf) {
+    return wg_block_jacobi<T, RPL, TW>(J, rows, ncols, rows, V, ncols, JACOBI_MAX_SWEEPS, gbuf);
+}
+
 template <typename T>
-__device__ void ph_svd(const RecompDesc<T> &d, T *lds, int lds_elems) {
+__device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
+    if (rows <= 64) return block_jacobi_nl<T, 1, 16>(J, rows, ncols, V, gbuf);
+    if (rows <= 128) return block_jacobi_nl<T, 2, 16>(J, rows, ncols, V, gbuf);
+    if (rows <= 256) return block_jacobi_nl<T, 4, 16>(J, rows, ncols, V, gbuf);
+    if (rows <= 384) return block_jacobi_nl<T, 6, 8>(J, rows, ncols, V, gbuf);
+    if (rows <= 512) return block_jacobi_nl<T, 8, 8>(J, rows, ncols, V, gbuf);
+    return block_jacobi_nl<T, 12, 8>(J, rows, ncols, V, gbuf);
+}
+
+constexpr int JACOBI_ROWS_MAX = 768;
+constexpr int JACOBI_LDS_ELEMS = block_jacobi_lds_elems<double, 16>() + 8;
+
+template <typename T>
+__device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
-    const int K = (int) w.hdr[HDR_K];
-    const int m = d.m, n = d.n;
-    const int mu = m < K ? m : K, mv = n < K ? n : K;
-    const int size_s = mu < mv ? mu : mv;
-    __shared__ int s_rot;
-    __shared__ int s_flag_nan;
-    // Stage X and W in LDS when both fit.
-    T *X = w.x, *W = w.w;
-    const bool in_lds = (int64_t) mv * mu + (int64_t) mu * mu <= lds_elems;
-    if (in_lds) {
-        X = lds;
-        W = lds + (int64_t) mv * mu;
-        for (int e = threadIdx.x; e < mv * mu; e += NT) X[e] = w.x[e];
-        for (int e = threadIdx.x; e < mu * mu; e += NT) W[e] = w.w[e];
+    const Dims z = dims_of(d, w);
+    const int mu = z.mu, kk = z.kk, K = z.K;
+    __shared__ int s_nan;
+    int sweeps = 0;
+    if (mu > JACOBI_ROWS_MAX) {
+        if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
+    } else {
+        sweeps = run_block_jacobi<T>(w.j, mu, kk, w.v1, lds);
     }
+    if (threadIdx.x == 0) s_nan = 0;
     __syncthreads();
-    const T tol = sqrt(T(mv)) * Eps<T>::value;
-    const int np = mu + (mu & 1);
     const int l = lane_id();
-    // Columns whose squared norm is below (eps * ||X||_F)^2 are numerically zero
-    // (they carry singular values at the rounding level of the largest one); pairs
-    // involving them are not rotated.  Without this floor a core with more columns
-    // than rows (K > n) never meets the relative orthogonality test.
-    __shared__ T s_red[NWAVES];
-    T fro_part = T(0);
-    for (int e = threadIdx.x; e < mv * mu; e += NT) fro_part += X[e] * X[e];
-    const T fro2 = wg_sum(fro_part, s_red);
-    const T floor2 = Eps<T>::value * Eps<T>::value * fro2;
-    int sweep = 0;
-    for (; sweep < JACOBI_MAX_SWEEPS && mu > 1; ++sweep) {
-        if (threadIdx.x == 0) s_rot = 0;
-        __syncthreads();
-        int rotated = 0;
-        for (int s = 0; s < np - 1; ++s) {
-            for (int pr = wave_id(); pr < np / 2; pr += NWAVES) {
-                int p, q;
-                if (pr == 0) {
-                    p = np - 1;
-                    q = s;
-                } else {
-                    p = (s + pr) % (np - 1);
-                    q = (s - pr + np - 1) % (np - 1);
-                }
-                if (p > q) {
-                    const int t = p;
-                    p = q;
-                    q = t;
-                }
-                if (q >= mu) continue;
-                T *xp = X + (int64_t) p * mv, *xq = X + (int64_t) q * mv;
-                T app = 0, aqq = 0, apq = 0;
-                for (int r = l; r < mv; r += WAVE) {
-                    const T a = xp[r], b = xq[r];
-                    app += a * a;
-                    aqq += b * b;
-                    apq += a * b;
-                }
-                app = wave_sum(app);
-                aqq = wave_sum(aqq);
-                apq = wave_sum(apq);
-                if (!(app > floor2) || !(aqq > floor2)) continue;
-                if (fabs(apq) <= tol * sqrt(app) * sqrt(aqq)) continue;
-                const T zeta = (aqq - app) / (T(2) * apq);
-                const T t = dsign(T(1), zeta) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
-                const T c = T(1) / sqrt(T(1) + t * t);
-                const T sn = c * t;
-                for (int r = l; r < mv; r += WAVE) {
-                    const T a = xp[r], b = xq[r];
-                    xp[r] = c * a - sn * b;
-                    xq[r] = sn * a + c * b;
-                }
-                T *wp = W + (int64_t) p * mu, *wq = W + (int64_t) q * mu;
-                for (int r = l; r < mu; r += WAVE) {
-                    const T a = wp[r], b = wq[r];
-                    wp[r] = c * a - sn * b;
-                    wq[r] = sn * a + c * b;
-                }
-                rotated = 1;
-            }
-            __syncthreads();
-        }
-        if (rotated && l == 0) atomicOr(&s_rot, 1);
-        __syncthreads();
-        if (s_rot == 0) break;
-        __syncthreads();
-    }
-    // sigma_j = |x_j|
-    if (threadIdx.x == 0) s_flag_nan = 0;
-    __syncthreads();
-    for (int j = wave_id(); j < mu; j += NWAVES) {
-        const T *xj = X + (int64_t) j * mv;
+    for (int j = wave_id(); j < kk; j += NWAVES) {
+        const T *zj = w.j + (int64_t) j * mu;
         T s = 0;
-        for (int r = l; r < mv; r += WAVE) s += xj[r] * xj[r];
+        for (int r = l; r < mu; r += WAVE) s += zj[r] * zj[r];
         s = wave_sum(s);
         if (l == 0) {
             w.sig[j] = sqrt(s);
-            if (!(s == s)) atomicOr(&s_flag_nan, 1);
+            if (!(s == s) || s > T(1e300)) atomicOr(&s_nan, 1);
         }
     }
     __syncthreads();
-    // descending stable sort by rank counting
-    for (int j = threadIdx.x; j < mu; j += NT) {
+    for (int j = threadIdx.x; j < kk; j += NT) {
         const T sj = w.sig[j];
         int pos = 0;
-        for (int k = 0; k < mu; ++k) {
+        for (int k = 0; k < kk; ++k) {
             const T sk = w.sig[k];
             pos += (sk > sj) || (sk == sj && k < j);
         }
@@ -547,7 +524,7 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds, int lds_elems) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        // rank rule on sorted sigma (first size_s values)
+        const int size_s = kk;
         int64_t rk;
         if (d.fixed_rank > 0) {
             rk = d.fixed_rank;
@@ -555,7 +532,7 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds, int lds_elems) {
             if (rk > size_s) rk = size_s;
         } else {
             rk = size_s;
-            const T s0 = w.sig[w.perm[0]];
+            const T s0 = size_s > 0 ? w.sig[w.perm[0]] : T(0);
             const T thr = d.truncated_svd ? T(d.accuracy) * s0 : T(d.accuracy);
             for (int i = 1; i < size_s; ++i) {
                 if (w.sig[w.perm[i]] < thr) {
@@ -566,44 +543,45 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds, int lds_elems) {
         }
         if (rk < 1) rk = 1;
         if (rk > d.max_rank) rk = d.max_rank;
+        if (rk > size_s && size_s > 0) rk = size_s;
         w.hdr[HDR_RANK] = rk;
         w.hdr[HDR_SIZES] = size_s;
-        w.hdr[HDR_SWEEPS] = sweep;
+        w.hdr[HDR_SWEEPS] = sweeps;
         int st = 0;
-        if (sweep >= JACOBI_MAX_SWEEPS) st |= STATUS_NOCONV;
-        if (s_flag_nan) st |= STATUS_NAN;
+        if (sweeps >= JACOBI_MAX_SWEEPS) st |= STATUS_NOCONV;
+        if (s_nan) st |= STATUS_NAN;
         w.hdr[HDR_STATUS] |= st;
         if (d.sigma_out) {
             for (int i = 0; i < size_s; ++i) d.sigma_out[i] = (double) w.sig[w.perm[i]];
         }
     }
-    if (in_lds) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < mv * mu; e += NT) w.x[e] = X[e];
-        for (int e = threadIdx.x; e < mu * mu; e += NT) w.w[e] = W[e];
-    }
     __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
-// Phase 5: reconstruction.  side 0: C_U[:, c0:c1] = Q_U [W(:, perm); 0];
-// side 1: vtmp[:, c0:c1] = Q_V [Z(:, perm); 0].  Columns [c0, c1) of the new
-// rank are handled by this workgroup.
+// Phase: reconstruction of columns [c0, c1) of the new rank.
+//   side 0: C_U[:, c] = Q_U [z_perm(c) / sigma; 0]
+//   side 1: vtmp[:, c] = Q_Y [v1_perm(c) * sigma; 0]
 // ---------------------------------------------------------------------------
 template <typename T, int PB>
 __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *lds) {
     Ws<T> w = ws_of(d);
-    const int K = (int) w.hdr[HDR_K];
+    const Dims z = dims_of(d, w);
     const int r = (int) w.hdr[HDR_RANK];
     if (c1 > r) c1 = r;
     if (c0 >= c1) return;
-    const int m = d.m, n = d.n;
-    const int mu = m < K ? m : K, mv = n < K ? n : K;
+    const int m = d.m, n = d.n, mu = z.mu, kk = z.kk;
     if (side == 0) {
         T *B = d.cdata;  // ld m
         for (int64_t e = threadIdx.x; e < (int64_t) m * (c1 - c0); e += NT) {
             const int i = (int) (e % m), c = c0 + (int) (e / m);
-            B[i + (int64_t) c * m] = (i < mu) ? w.w[i + (int64_t) w.perm[c] * mu] : T(0);
+            T v = T(0);
+            if (i < mu && c < kk) {
+                const int j = w.perm[c];
+                const T sg = w.sig[j];
+                v = (sg > T(0)) ? w.j[i + (int64_t) j * mu] / sg : T(0);
+            }
+            B[i + (int64_t) c * m] = v;
         }
         __syncthreads();
         wg_apply_q<T, PB>(m, mu, w.us, m, w.t_u, B, m, c0, c1, lds);
@@ -611,14 +589,19 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
         T *B = w.vtmp;  // ld n
         for (int64_t e = threadIdx.x; e < (int64_t) n * (c1 - c0); e += NT) {
             const int i = (int) (e % n), c = c0 + (int) (e / n);
-            B[i + (int64_t) c * n] = (i < mv) ? w.x[i + (int64_t) w.perm[c] * mv] : T(0);
+            T v = T(0);
+            if (i < kk && c < kk) {
+                const int j = w.perm[c];
+                v = w.v1[i + (int64_t) j * kk] * w.sig[j];
+            }
+            B[i + (int64_t) c * n] = v;
         }
         __syncthreads();
-        wg_apply_q<T, PB>(n, mv, w.vs, n, w.t_v, B, n, c0, c1, lds);
+        wg_apply_q<T, PB>(n, kk, w.y, n, w.t_y, B, n, c0, c1, lds);
     }
 }
 
-// Phase 6: C_V (r x n, ld r) = vtmp^T; publish the new rank.
+// Phase: C_V (r x n, ld r) = vtmp^T.
 template <typename T>
 __device__ void ph_finish(const RecompDesc<T> &d, int wg, int nwg) {
     Ws<T> w = ws_of(d);

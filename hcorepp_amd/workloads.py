@@ -76,16 +76,35 @@ def loguniform_ranks(rng, n, lo=8, hi=192):
     return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n))).astype(np.int64)
 
 
-def batch_ccc(n: int, nb: int, seed: int, lo=8, hi=192, decay=0.85):
-    """Independent CCC updates: returns list of dicts with (au, av, bu, bv, cu, cv)."""
+def _ccc_update(seed: int, i: int, nb: int, ra: int, rb: int, rc: int, decay: float):
+    rng = np.random.default_rng([seed, i])
+    au, av = lowrank_factors(rng, nb, nb, ra, decay)
+    bu, bv = lowrank_factors(rng, nb, nb, rb, decay)
+    cu, cv = lowrank_factors(rng, nb, nb, rc, decay)
+    return dict(au=au, av=av, bu=bu, bv=bv, cu=cu, cv=cv)
+
+
+def batch_ccc(n: int, nb: int, seed: int, lo=8, hi=192, decay=0.85, threads: int = 8):
+    """Independent CCC updates: list of dicts with (au, av, bu, bv, cu, cv).  Each factor pair
+    is U = orth(G_{nb x r}), V = diag(decay^i) orth(G_{nb x r})^T with r log-uniform in [lo, hi].
+    Update i draws from its own generator seeded with (seed, i), so the result does not depend
+    on the thread count."""
+    import concurrent.futures as cf
     rng = np.random.default_rng(seed)
     ra = loguniform_ranks(rng, n, lo, hi)
     rb = loguniform_ranks(rng, n, lo, hi)
     rc = loguniform_ranks(rng, n, lo, hi)
-    out = []
-    for i in range(n):
-        au, av = lowrank_factors(rng, nb, nb, int(ra[i]), decay)
-        bu, bv = lowrank_factors(rng, nb, nb, int(rb[i]), decay)
-        cu, cv = lowrank_factors(rng, nb, nb, int(rc[i]), decay)
-        out.append(dict(au=au, av=av, bu=bu, bv=bv, cu=cu, cv=cv))
-    return out
+    args = [(seed, i, nb, int(ra[i]), int(rb[i]), int(rc[i]), decay) for i in range(n)]
+    if threads <= 1 or n < 16:
+        return [_ccc_update(*a) for a in args]
+    try:
+        from threadpoolctl import threadpool_limits
+        limit = threadpool_limits(limits=1)
+    except Exception:  # pragma: no cover - optional dependency
+        limit = None
+    try:
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            return list(ex.map(lambda a: _ccc_update(*a), args))
+    finally:
+        if limit is not None:
+            limit.restore_original_limits()

@@ -98,6 +98,8 @@ size_t hcorepp_ctx_arena_bytes(hcorepp_ctx *ctx);
 hcorepp_status hcorepp_malloc(size_t bytes, void **out);
 hcorepp_status hcorepp_free(void *ptr);
 hcorepp_status hcorepp_memcpy(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Stream-ordered copy on the context stream, no synchronisation. */
+hcorepp_status hcorepp_memcpy_async(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes);
 hcorepp_status hcorepp_memset(hcorepp_ctx *ctx, void *dst, int value, size_t bytes);
 
 /* Stream-ordered timing on the context stream: record event `slot` (0..15) and read the
@@ -165,6 +167,10 @@ hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps);
 /* Bytes of device workspace the batch holds. */
 size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b);
 hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b);
+
+/* Diagnostics (no counterpart in the reference): copy the device workspace of the last plain
+ * recompression on `ctx` to host memory and describe its layout in info[16]. */
+int64_t hcorepp_debug_workspace(hcorepp_ctx *ctx, void *host, size_t cap, int64_t *info);
 
 /* Human-readable description of the last error on this thread. */
 const char *hcorepp_last_error(void);
