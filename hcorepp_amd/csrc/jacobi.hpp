@@ -10,7 +10,9 @@
 // ordering; step 0 of every sweep rotates all pairs inside the tile (so the
 // intra-block pairs of every block are visited once per sweep), later steps
 // only the cross pairs.  Rotations of a tile are accumulated in a TW x TW
-// matrix G (LDS, per wave) and applied to V's block columns afterwards.
+// matrix G whose row l lives in lane l's registers (every lane applies the
+// broadcast rotation to its own row); G goes through LDS once per tile and is
+// applied to V's block columns afterwards.
 //
 // Convergence test: |z_p . z_q| <= sqrt(rows) * eps * |z_p| |z_q| (relative,
 // as LAPACK's one-sided Jacobi), pairs involving a column whose norm is below
@@ -52,7 +54,7 @@ __host__ __device__ __forceinline__ constexpr int jpair_q(int rd, int k) {
 
 // One round: TW/2 disjoint pairs of the register tile y.
 template <typename T, int RPL, int TW, bool FULL>
-__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], int rd, T tol, T floor2, T *G) {
+__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&g)[TW], int rd, T tol, T floor2) {
     constexpr int NP = TW / 2;
     constexpr int NV = (3 * NP <= 16) ? 16 : 32;
     const int l = lane_id();
@@ -95,26 +97,11 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], int rd, T tol, T fl
                 y[t][p] = ck * a - sk * b;
                 y[t][q] = sk * a + ck * b;
             }
+            // lane-local row of G <- G * R_pq (lane l < TW owns row l of G)
+            const T gp = g[p], gq = g[q];
+            g[p] = ck * gp - sk * gq;
+            g[q] = sk * gp + ck * gq;
             any = 1;
-        }
-    }
-    if (any) {
-        // G <- G * R_pq for every rotated pair: lanes (slot, row) with TW rows per slot.
-        constexpr int SLOTS = WAVE / TW;
-#pragma unroll
-        for (int it = 0; it < (NP + SLOTS - 1) / SLOTS; ++it) {
-            const int k = l / TW + it * SLOTS;
-            const int row = l % TW;
-            const int ksafe = k < NP ? k : 0;
-            const T ck = __shfl(c, ksafe, WAVE);
-            const T sk = __shfl(s, ksafe, WAVE);
-            if (k < NP && sk != T(0)) {
-                const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
-                const T a = G[row + TW * p], b = G[row + TW * q];
-                G[row + TW * p] = ck * a - sk * b;
-                G[row + TW * q] = sk * a + ck * b;
-            }
-            __builtin_amdgcn_wave_barrier();
         }
     }
     return any;
@@ -137,17 +124,27 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
             y[t][c] = (r < rows && gc < ncols) ? J[r + (int64_t) gc * ldj] : T(0);
         }
     }
-    for (int e = l; e < TW * TW; e += WAVE) G[e] = (e % (TW + 1) == 0) ? T(1) : T(0);
-    __builtin_amdgcn_wave_barrier();
+    T g[TW];
+#pragma unroll
+    for (int c = 0; c < TW; ++c) g[c] = (c == l) ? T(1) : T(0);
     int any = 0;
     if (full) {
 #pragma unroll
-        for (int rd = 0; rd < TW - 1; ++rd) any |= jacobi_round<T, RPL, TW, true>(y, rd, tol, floor2, G);
+        for (int rd = 0; rd < TW - 1; ++rd) any |= jacobi_round<T, RPL, TW, true>(y, g, rd, tol, floor2);
     } else {
 #pragma unroll
-        for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, TW, false>(y, rd, tol, floor2, G);
+        for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, TW, false>(y, g, rd, tol, floor2);
     }
     if (any) {
+        // publish G (row l from lane l) to the wave's LDS slot; wavefront-scope fences order the
+        // cross-lane exchange
+        if (l < TW) {
+#pragma unroll
+            for (int c = 0; c < TW; ++c) G[l + TW * c] = g[c];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int c = 0; c < TW; ++c) {
             const int gc = (c < B) ? cI + c : cJ + (c - B);

@@ -115,8 +115,9 @@ hcorepp_status hcorepp_ctx_event_elapsed(hcorepp_ctx *ctx, int slot0, int slot1,
  *  - All 8 dense/compressed combinations.  Compressed C requires opA = opB = 'N' (the
  *    reference's CompressedTile::Gemm ignores the ops, Compressed.cpp:331-380); other ops
  *    return HCOREPP_ERR_UNSUPPORTED.  Dense C supports 'N' and 'T'.
- *  - cholesky != 0 selects the reference's aCholesky operand plumbing (HCore.cpp:160-174):
- *    for compressed A and B with opA='N', opB='T' it computes C = alpha*A*B^T + beta*C.
+ *  - cholesky != 0 (the reference's aCholesky operand re-plumbing, HCore.cpp:160-174, used
+ *    only by its TLR Cholesky driver) returns HCOREPP_ERR_UNSUPPORTED: the Cholesky tile
+ *    operations are not part of this library.
  *  - Plain form: the recompression runs as a chain of launches on the context stream and the
  *    new rank is read back with one stream synchronisation, as the reference does
  *    (src/kernels/cuda/CudaKernels.cu:656-697).  C->rank is updated on return.

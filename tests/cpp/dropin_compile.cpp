@@ -1,0 +1,50 @@
+// Compiles a caller written against the reference's C++ API (HCore<T>::Gemm with
+// DenseTile / CompressedTile / CompressionParameters / RunContext / MemoryUnit, as in
+// the reference's tests/api/TestHCore.cpp) against the drop-in headers.  Built and
+// linked on the CPU; it runs GPU work only when invoked with the argument "run".
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include <hcorepp/api/HCore.hpp>
+
+using namespace hcorepp::api;
+using namespace hcorepp::operators;
+using namespace hcorepp::kernels;
+using namespace hcorepp::dataunits;
+
+int main(int argc, char **argv) {
+    if (argc < 2 || std::strcmp(argv[1], "run") != 0) {
+        std::printf("dropin_compile: built (%s)\n", hcorepp_version());
+        return 0;
+    }
+    // Compressed A (2x1 rank 1), B (1x2 as U 1x1, V 1x2), C dense 2x2: C = A*B + C.
+    const size_t m = 4, n = 4, r = 1;
+    std::vector<double> a(m * r + r * n), b(m * r + r * n), c(m * n, 0.0);
+    for (size_t i = 0; i < a.size(); ++i) a[i] = 1.0 + i;
+    for (size_t i = 0; i < b.size(); ++i) b[i] = 0.5 * (i + 1);
+    RunContext ctx;
+    CompressedTile<double> A(m, n, a.data(), m, r, ctx);
+    CompressedTile<double> B(m, n, b.data(), m, r, ctx);
+    DenseTile<double> C(m, n, c.data(), m, ctx);
+    size_t flops = 0;
+    MemoryUnit<double> &unit = MemoryHandler<double>::GetInstance().GetMemoryUnit();
+    HCore<double>::Gemm(1.0, A, blas::Op::NoTrans, B, blas::Op::NoTrans, 0.0, C, ctx, flops, unit,
+                        CompressionParameters(1e-9));
+    ctx.Sync();
+    std::vector<double> out(m * n);
+    ThrowIfError(hcorepp_memcpy(ctx.Handle(), out.data(), C.GetTileSubMatrix(0), out.size() * sizeof(double)));
+    // expected: (Au Av)(Bu Bv)
+    double err = 0;
+    for (size_t j = 0; j < n; ++j)
+        for (size_t i = 0; i < m; ++i) {
+            double s = 0;
+            for (size_t k = 0; k < m; ++k) {
+                const double av = a[i] * a[m + k], bv = b[k] * b[m + j];
+                s += av * bv;
+            }
+            err = std::max(err, std::abs(out[i + m * j] - s));
+        }
+    std::printf("dropin_compile: max abs err %.3e flops %zu\n", err, flops);
+    return err < 1e-10 ? 0 : 1;
+}

@@ -54,3 +54,40 @@ def test_ops_fail_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(H, "_lib", None)
     with pytest.raises(RuntimeError):
         H.lib()
+
+
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_compile.cpp")
+
+
+def build_dropin(out_dir):
+    """Compile a caller written against the reference's C++ API (HCore<T>::Gemm,
+    DenseTile, CompressedTile, CompressionParameters, RunContext, MemoryUnit) against the
+    drop-in headers in include/hcorepp and link it to the HIP library."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = os.path.join(str(out_dir), "dropin_compile")
+    libdir = os.path.join(ROOT, "hcorepp_amd")
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), DROPIN_SRC,
+                    "-L", libdir, "-lhcorepp_amd", "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    return exe
+
+
+def test_cpp_dropin_headers_compile_and_link(tmp_path):
+    import subprocess
+    import hcorepp_amd as H
+    H.lib()
+    exe = build_dropin(tmp_path)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    assert "built" in out
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_gemm_runs(tmp_path):
+    import subprocess
+    exe = build_dropin(tmp_path)
+    res = subprocess.run([exe, "run"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "max abs err" in res.stdout
