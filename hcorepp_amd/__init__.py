@@ -58,7 +58,7 @@ EXPORTED = [
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
-    "hcorepp_batch_workspace_bytes", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
+    "hcorepp_batch_workspace_bytes", "hcorepp_batch_phase_times", "hcorepp_batch_set_grid", "hcorepp_batch_get_grid", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
     "hcorepp_version",
 ]
 
@@ -121,6 +121,10 @@ def lib():
     L.hcorepp_batch_ranks.argtypes = [P, P]
     L.hcorepp_batch_set_ranks.argtypes = [P, P]
     L.hcorepp_batch_status.argtypes = [P, C.POINTER(I64)]
+    L.hcorepp_batch_phase_times.argtypes = [P, P]
+    L.hcorepp_batch_set_grid.argtypes = [P, C.c_int]
+    L.hcorepp_batch_get_grid.argtypes = [P]
+    L.hcorepp_batch_get_grid.restype = C.c_int
     L.hcorepp_batch_workspace_bytes.argtypes = [P]
     L.hcorepp_batch_workspace_bytes.restype = C.c_size_t
     L.hcorepp_debug_workspace.argtypes = [P, P, C.c_size_t, P]
@@ -433,6 +437,20 @@ class Batch:
 
     def workspace_bytes(self) -> int:
         return lib().hcorepp_batch_workspace_bytes(self.h)
+
+    def set_grid(self, workgroups: int):
+        """Number of workgroups of the persistent kernel (drops a captured graph)."""
+        _check(lib().hcorepp_batch_set_grid(self.h, int(workgroups)))
+
+    def grid(self) -> int:
+        return lib().hcorepp_batch_get_grid(self.h)
+
+    def phase_times(self) -> np.ndarray:
+        """Per-update phase timestamps of the last pass (n x 8, 100 MHz ticks); needs a
+        library built with HCAMD_PHASE_TIMING."""
+        out = np.zeros((self.n, 8), dtype=np.int64)
+        _check(lib().hcorepp_batch_phase_times(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def sync_ranks_to_tiles(self):
         rk = self.ranks()

@@ -39,7 +39,8 @@ hcorepp_status fail(hcorepp_status st, const char *fmt, ...) {
 
 // Dynamic LDS per workgroup for the QR / SVD / reconstruction / batched kernels:
 // 80 KB lets two workgroups share one CU's 160 KB.
-constexpr int64_t kLdsBytes = 80 * 1024;
+// Dynamic LDS per workgroup: two workgroups of the batched kernel share one CU's 160 KB.
+constexpr int64_t kLdsBytes = 78 * 1024;
 
 }  // namespace
 
@@ -396,6 +397,7 @@ struct hcorepp_batch {
     size_t ws_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    std::vector<int64_t> hdr_off;  // byte offset of each update's workspace header in ws
 };
 
 extern "C" {
@@ -681,6 +683,8 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
             return cleanup(fail(HCOREPP_ERR_INVALID, "operand shapes do not match C (update %lld)", (long long) i));
         e.rd.crank = bt->ranks + i;
         ranks[(size_t) i] = C[i].rank;
+        bt->hdr_off.push_back((int64_t) ((char *) e.rd.ws - bt->ws) +
+                              (int64_t) ws_layout<T>(e.rd.m, e.rd.n, e.rd.kmax, e.rd.max_rank).hdr * (int64_t) sizeof(T));
     }
     std::vector<int32_t> order((size_t) n);
     for (int64_t i = 0; i < n; ++i) order[(size_t) i] = (int32_t) i;
@@ -706,6 +710,35 @@ hcorepp_status hcorepp_batch_run(hcorepp_batch *b) {
     HIP_TRY(launch_batched<double>(s, (const BatchEntry<double> *) b->entries, b->order, b->n, b->ctr, b->grid,
                                    kLdsBytes, b->pb));
     return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_set_grid(hcorepp_batch *b, int workgroups) {
+    if (!b || workgroups < 1) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    b->grid = (int) std::min<int64_t>(workgroups, std::max<int64_t>(b->n, 1));
+    if (b->graph_exec) {
+        hipGraphExecDestroy(b->graph_exec);
+        b->graph_exec = nullptr;
+    }
+    if (b->graph) {
+        hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return HCOREPP_OK;
+}
+
+int hcorepp_batch_get_grid(hcorepp_batch *b) { return b ? b->grid : 0; }
+
+hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times) {
+    if (!b || (!times && b->n > 0)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+#ifndef HCAMD_PHASE_TIMING
+    return fail(HCOREPP_ERR_UNSUPPORTED, "library built without HCAMD_PHASE_TIMING");
+#else
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    for (int64_t i = 0; i < b->n; ++i)
+        HIP_TRY(hipMemcpy(times + 8 * i, b->ws + b->hdr_off[(size_t) i] + HDR_T0 * sizeof(int64_t),
+                          8 * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return HCOREPP_OK;
+#endif
 }
 
 hcorepp_status hcorepp_batch_capture_graph(hcorepp_batch *b) {

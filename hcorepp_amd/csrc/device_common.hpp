@@ -91,6 +91,139 @@ __device__ __forceinline__ T wave_multi_sum(T (&v)[NV]) {
     return v[0];
 }
 
+// ---------------------------------------------------------------------------
+// Cross-lane exchanges on 32- or 64-bit values without the LDS crossbar: DPP row
+// permutations (row_mirror, row_half_mirror, quad_perm) and gfx950's permlane swaps.
+// ---------------------------------------------------------------------------
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ double dpp_update(double old, double src) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(src), CTRL, ROW_MASK, BANK_MASK,
+                                               false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, ROW_MASK, BANK_MASK,
+                                               false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ float dpp_update(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, ROW_MASK,
+                                                      BANK_MASK, false));
+}
+
+// v_permlane32_swap / v_permlane16_swap on (a, b): with W = 32 (16), the upper half-wave (odd
+// rows) of a is exchanged with the lower half-wave (even rows) of b.  Afterwards
+// a = [a_lo | b_lo] and b = [a_hi | b_hi] per lane block, so a + b sums each value with its
+// partner's copy, the lower block keeping value a and the upper block value b.
+template <int W>
+__device__ __forceinline__ void permlane_swap(double &a, double &b) {
+    if constexpr (W == 32) {
+        auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(a), __double2loint(b), false, false);
+        auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(a), __double2hiint(b), false, false);
+        a = __hiloint2double((int) hi[0], (int) lo[0]);
+        b = __hiloint2double((int) hi[1], (int) lo[1]);
+    } else {
+        auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(a), __double2loint(b), false, false);
+        auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(a), __double2hiint(b), false, false);
+        a = __hiloint2double((int) hi[0], (int) lo[0]);
+        b = __hiloint2double((int) hi[1], (int) lo[1]);
+    }
+}
+template <int W>
+__device__ __forceinline__ void permlane_swap(float &a, float &b) {
+    if constexpr (W == 32) {
+        auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(a), __float_as_int(b), false, false);
+        a = __int_as_float((int) r[0]);
+        b = __int_as_float((int) r[1]);
+    } else {
+        auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(a), __float_as_int(b), false, false);
+        a = __int_as_float((int) r[0]);
+        b = __int_as_float((int) r[1]);
+    }
+}
+
+constexpr int DPP_QUAD_XOR1 = 0xB1;    // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;    // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_MIRROR = 0x140;  // lane i <- lane 15-i within a row of 16
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;  // lane i <- lane 7-i within each half row
+
+// One reduce-scatter level of the wave multi-value sum on bit BIT of the lane id: on return
+// v[i] (i < CNT/2) holds, for lanes with that bit clear, the pair sum of value i, and for
+// lanes with it set, the pair sum of value i + CNT/2.
+template <typename T, int NV, int CNT, int BIT>
+__device__ __forceinline__ void multi_sum_level(T (&v)[NV], int l) {
+    constexpr int H = CNT / 2;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        T a = v[i], b = v[i + H];
+        if constexpr (BIT == 5) {
+            permlane_swap<32>(a, b);
+            v[i] = a + b;
+        } else if constexpr (BIT == 4) {
+            permlane_swap<16>(a, b);
+            v[i] = a + b;
+        } else if constexpr (BIT == 3) {
+            // lanes 0-7 of a row take the partner's a; lanes 8-15 the partner's b
+            const T x = dpp_update<DPP_ROW_MIRROR, 0xF, 0x3>(b, a);
+            const T y = dpp_update<DPP_ROW_MIRROR, 0xF, 0xC>(a, b);
+            v[i] = x + y;
+        } else if constexpr (BIT == 2) {
+            const T x = dpp_update<DPP_ROW_HALF_MIRROR, 0xF, 0x5>(b, a);
+            const T y = dpp_update<DPP_ROW_HALF_MIRROR, 0xF, 0xA>(a, b);
+            v[i] = x + y;
+        } else {
+            constexpr int CTRL = BIT == 1 ? DPP_QUAD_XOR2 : DPP_QUAD_XOR1;
+            const T ta = dpp_update<CTRL, 0xF, 0xF>(a, a);
+            const T tb = dpp_update<CTRL, 0xF, 0xF>(b, b);
+            v[i] = (l & (1 << BIT)) ? b + tb : a + ta;
+        }
+    }
+}
+
+template <typename T, int NV, int CNT, int BIT>
+struct MultiSumFast {
+    static __device__ __forceinline__ void run(T (&v)[NV], int l) {
+        if constexpr (BIT >= 0) {
+            if constexpr (CNT > 1) {
+                multi_sum_level<T, NV, CNT, BIT>(v, l);
+                MultiSumFast<T, NV, CNT / 2, BIT - 1>::run(v, l);
+            } else {
+                // one value left: plain butterfly sum on the remaining (low) lane bits
+                constexpr int CTRL = BIT == 1 ? DPP_QUAD_XOR2 : DPP_QUAD_XOR1;
+                if constexpr (BIT <= 1) {
+                    v[0] += dpp_update<CTRL, 0xF, 0xF>(v[0], v[0]);
+                } else if constexpr (BIT == 2) {
+                    v[0] += dpp_update<DPP_ROW_HALF_MIRROR, 0xF, 0xF>(v[0], v[0]);
+                } else if constexpr (BIT == 3) {
+                    v[0] += dpp_update<DPP_ROW_MIRROR, 0xF, 0xF>(v[0], v[0]);
+                } else {
+                    T a = v[0], b = v[0];
+                    permlane_swap<BIT == 5 ? 32 : 16>(a, b);
+                    v[0] = a + b;
+                }
+                MultiSumFast<T, NV, 1, BIT - 1>::run(v, l);
+            }
+        }
+    }
+};
+
+// Same result convention as wave_multi_sum (lane l holds the total of value index
+// l >> (6 - log2 NV)), computed with permlane swaps and DPP instead of ds_bpermute.
+template <typename T, int NV>
+__device__ __forceinline__ T wave_multi_sum_fast(T (&v)[NV]) {
+    static_assert((NV & (NV - 1)) == 0 && NV <= 64, "NV must be a power of two <= 64");
+    MultiSumFast<T, NV, NV, 5>::run(v, lane_id());
+    return v[0];
+}
+
+// Value of lane `src` (a compile-time or wave-uniform lane index) in every lane.
+__device__ __forceinline__ double read_lane(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float read_lane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 template <int NV>
 __host__ __device__ __forceinline__ constexpr int log2_const() {
     return NV <= 1 ? 0 : 1 + log2_const<NV / 2>();

@@ -52,7 +52,30 @@ __host__ __device__ __forceinline__ constexpr int jpair_q(int rd, int k) {
     }
 }
 
-// One round: TW/2 disjoint pairs of the register tile y.
+// Rotation (c, s) that annihilates apq for the Gram entries (app, aqq, apq) of a column pair,
+// or (1, 0) when the pair is already orthogonal to tolerance or involves a numerically zero
+// column.  t = tan(theta) is the smaller root of t^2 + 2 zeta t - 1 = 0 with
+// zeta = (aqq - app) / (2 apq), written as t = sign(d) e / (|d| + hypot(d, e)) for d = aqq - app,
+// e = 2 apq (one division and one square root); c = 1/sqrt(1 + t^2), s = c t.
+template <typename T>
+__device__ __forceinline__ void jacobi_rotation(T app, T aqq, T apq, T tol, T floor2, T &c, T &s) {
+    c = T(1);
+    s = T(0);
+    if (app > floor2 && aqq > floor2 && fabs(apq) > tol * sqrt(app * aqq)) {
+        const T d = aqq - app, e = T(2) * apq;
+        // scale by a power of two so that d*d + e*e neither overflows nor underflows
+        const int ex = ilogb(fmax(fabs(d), fabs(e)));
+        const T ds = scalbn(d, -ex), es = scalbn(e, -ex);
+        const T h = sqrt(ds * ds + es * es);
+        const T t = (d >= T(0) ? es : -es) / (fabs(ds) + h);
+        c = T(1) / sqrt(T(1) + t * t);
+        s = c * t;
+    }
+}
+
+// One round: TW/2 disjoint pairs of the register tile y.  The 3*TW/2 dot products are
+// reduce-scattered across the wave with permlane/DPP exchanges; lane k < TW/2 gathers the three
+// sums of pair k, computes its rotation, and every lane reads the rotations back with readlane.
 template <typename T, int RPL, int TW, bool FULL>
 __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&g)[TW], int rd, T tol, T floor2) {
     constexpr int NP = TW / 2;
@@ -72,30 +95,25 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&g)[TW], int rd,
             part[3 * k + 2] += a * b;
         }
     }
-    const T tot = wave_multi_sum<T, NV>(part);
+    const T tot = wave_multi_sum_fast<T, NV>(part);
     const int kk = l < NP ? l : 0;
     const T app = __shfl(tot, multi_sum_src_lane<NV>(3 * kk), WAVE);
     const T aqq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 1), WAVE);
     const T apq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 2), WAVE);
-    T c = T(1), s = T(0);
-    if (l < NP && app > floor2 && aqq > floor2 && fabs(apq) > tol * sqrt(app) * sqrt(aqq)) {
-        const T zeta = (aqq - app) / (T(2) * apq);
-        const T t = dsign(T(1), zeta) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
-        c = T(1) / sqrt(T(1) + t * t);
-        s = c * t;
-    }
+    T c, s;
+    jacobi_rotation(app, aqq, apq, tol, floor2, c, s);
     int any = 0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const T ck = __shfl(c, k, WAVE);
-        const T sk = __shfl(s, k, WAVE);
+        const T sk = read_lane(s, k);
         if (sk != T(0)) {
+            const T ck = read_lane(c, k);
             const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
 #pragma unroll
-            for (int t = 0; t < RPL; ++t) {
-                const T a = y[t][p], b = y[t][q];
-                y[t][p] = ck * a - sk * b;
-                y[t][q] = sk * a + ck * b;
+            for (int r = 0; r < RPL; ++r) {
+                const T a = y[r][p], b = y[r][q];
+                y[r][p] = ck * a - sk * b;
+                y[r][q] = sk * a + ck * b;
             }
             // lane-local row of G <- G * R_pq (lane l < TW owns row l of G)
             const T gp = g[p], gq = g[q];
@@ -157,6 +175,8 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
         __builtin_amdgcn_wave_barrier();
         // V[:, tile] <- V[:, tile] * G   (V is ncols x ncols)
         for (int r = l; r < ncols; r += WAVE) {
+            // re-read G from LDS every row batch instead of holding its TW*TW values in registers
+            __asm__ volatile("" ::: "memory");
             T v[TW];
 #pragma unroll
             for (int c = 0; c < TW; ++c) {

@@ -6,54 +6,64 @@
 
 namespace hcamd {
 
+// Every engine kernel runs 256 threads and is register-capped for two waves per SIMD
+// (256 VGPRs per lane), so that two workgroups of the batched kernel share a CU: the
+// latency-bound Jacobi rounds and panel reductions of one overlap the other's.  The
+// device functions the kernels share inherit the cap only when every caller has it.
+#define HCAMD_KERNEL __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+
+// Phase timestamps of the batched kernel (100 MHz constant clock), recorded only in builds
+// with HCAMD_PHASE_TIMING; read back with hcorepp_batch_phase_times.
+#ifdef HCAMD_PHASE_TIMING
+#define HCAMD_TSTAMP() ((uint64_t) wall_clock64())
+#else
+#define HCAMD_TSTAMP() ((uint64_t) 0)
+#endif
+
 template <typename T>
-__global__ void __launch_bounds__(NT) k_gemm(int M, int N, int Kd, T alpha, const T *A, int64_t lda, int ta,
+__global__ void HCAMD_KERNEL k_gemm(int M, int N, int Kd, T alpha, const T *A, int64_t lda, int ta,
                                              const T *B, int64_t ldb, int tb, T beta, T *C, int64_t ldc) {
     __shared__ T lds[GT_LDS_ELEMS];
     wg_gemm_tile<T>(M, N, Kd, alpha, A, lda, ta != 0, B, ldb, tb != 0, beta, C, ldc, blockIdx.x, blockIdx.y, lds);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_stack(RecompDesc<T> d) {
+__global__ void HCAMD_KERNEL k_stack(RecompDesc<T> d) {
     ph_stack<T>(d, blockIdx.x, gridDim.x);
 }
 
 template <typename T, int PB>
-__global__ void __launch_bounds__(NT) k_qr(RecompDesc<T> d, int which) {
+__global__ void HCAMD_KERNEL k_qr(RecompDesc<T> d, int which) {
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     if (which == 0) wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds);
-    else wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds);
+    else ph_qr_y<T, PB>(d, lds);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_y(RecompDesc<T> d) {
+__global__ void HCAMD_KERNEL k_y(RecompDesc<T> d) {
     __shared__ T lds[GT_LDS_ELEMS];
     ph_y<T>(d, blockIdx.x, gridDim.x, lds);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_jmat(RecompDesc<T> d) {
-    ph_jmat<T>(d, blockIdx.x, gridDim.x);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(NT) k_svd(RecompDesc<T> d) {
+__global__ void HCAMD_KERNEL k_svd(RecompDesc<T> d) {
     __shared__ T gbuf[JACOBI_LDS_ELEMS];
+    ph_core<T>(d, gbuf);
     ph_svd<T>(d, gbuf);
 }
 
 template <typename T, int PB>
-__global__ void __launch_bounds__(NT) k_recon(RecompDesc<T> d, int cols_per_wg) {
+__global__ void HCAMD_KERNEL k_recon(RecompDesc<T> d, int cols_per_wg) {
     extern __shared__ unsigned char smem_raw[];
     const int c0 = blockIdx.x * cols_per_wg;
     ph_recon<T, PB>(d, blockIdx.y, c0, c0 + cols_per_wg, reinterpret_cast<T *>(smem_raw));
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_finish(RecompDesc<T> d) {
+__global__ void HCAMD_KERNEL k_finish(RecompDesc<T> d) {
     ph_finish<T>(d, blockIdx.x, gridDim.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Ws<T> w = ws_of(d);
@@ -68,7 +78,7 @@ __global__ void k_set_i64(int64_t *p, int64_t v) {
 // Fill V (n x n, ld n) with the identity (FillIdentityMatrix, reference omp/kernels.cpp:144-151,
 // on a zeroed buffer) and copy U = W (m x n): the DDC ReadjustTile (Compressed.cpp:696-734).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_readjust_ddc(const T *W, int64_t m, int64_t n, T *cdata) {
+__global__ void HCAMD_KERNEL k_readjust_ddc(const T *W, int64_t m, int64_t n, T *cdata) {
     const int64_t tot = m * n + n * n;
     for (int64_t e = (int64_t) blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t) gridDim.x * NT) {
         if (e < m * n) {
@@ -93,7 +103,7 @@ __device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds) {
 }
 
 template <typename T, int PB>
-__global__ void __launch_bounds__(NT) k_batched(const BatchEntry<T> *entries, const int32_t *order, int64_t n,
+__global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32_t *order, int64_t n,
                                                 BatchCounters *ctr, int lds_elems) {
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
@@ -106,33 +116,48 @@ __global__ void __launch_bounds__(NT) k_batched(const BatchEntry<T> *entries, co
         __syncthreads();
         if (item >= n) break;
         const BatchEntry<T> &e = entries[order[item]];
-        run_temp_gemms<T>(e, lds);
         const RecompDesc<T> &d = e.rd;
+        const uint64_t t_start = HCAMD_TSTAMP();
+        run_temp_gemms<T>(e, lds);
+        const uint64_t t_gemm = HCAMD_TSTAMP();
         ph_stack<T>(d, 0, 1);
         __syncthreads();
         Ws<T> w = ws_of(d);
         const Dims z = dims_of(d, w);
+        uint64_t t_qru = 0, t_y = 0, t_qry = 0, t_svd = 0, t_ru = 0;
         if (z.K <= d.kmax) {
             wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds);
             __syncthreads();
+            t_qru = HCAMD_TSTAMP();
             ph_y<T>(d, 0, 1, lds);
             __syncthreads();
-            wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds);
+            t_y = HCAMD_TSTAMP();
+            ph_qr_y<T, PB>(d, lds);
             __syncthreads();
-            ph_jmat<T>(d, 0, 1);
-            __syncthreads();
+            ph_core<T>(d, lds);
+            t_qry = HCAMD_TSTAMP();
             ph_svd<T>(d, lds);
             __syncthreads();
+            t_svd = HCAMD_TSTAMP();
         }
         const int r = (z.K <= d.kmax) ? (int) w.hdr[HDR_RANK] : (int) *d.crank;
         if (z.K <= d.kmax) {
             ph_recon<T, PB>(d, 0, 0, r, lds);
             __syncthreads();
+            t_ru = HCAMD_TSTAMP();
             ph_recon<T, PB>(d, 1, 0, r, lds);
             __syncthreads();
             ph_finish<T>(d, 0, 1);
             __syncthreads();
         }
+#ifdef HCAMD_PHASE_TIMING
+        if (threadIdx.x == 0) {
+            const uint64_t ts[8] = {t_start, t_gemm, t_qru, t_y, t_qry, t_svd, t_ru, HCAMD_TSTAMP()};
+            for (int k = 0; k < 8; ++k) w.hdr[HDR_T0 + k] = (int64_t) ts[k];
+        }
+#else
+        (void) t_start; (void) t_gemm; (void) t_qru; (void) t_y; (void) t_qry; (void) t_svd; (void) t_ru;
+#endif
         if (threadIdx.x == 0) {
             *d.crank = r;
             const int64_t st = w.hdr[HDR_STATUS];
@@ -164,13 +189,13 @@ hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A,
     return hipGetLastError();
 }
 
+// Householder panel width for a QR of `rows` rows: the widest panel whose register tile in
+// apply_wy (rows/256 row chunks x PB per lane) stays within 32 values and whose LDS fits.
 int pick_pb(int64_t rows, int64_t lds_elems) {
-    const int pbs[3] = {16, 8, 4};
-    if (rows > 2048) return 0;  // apply_wy keeps at most 8 row chunks per lane
-    for (int i = 0; i < 3; ++i) {
-        const int pb = pbs[i];
-        if (qr_lds_elems(rows, pb) <= lds_elems) return pb;
-    }
+    if (rows > 2048) return 0;
+    const int pb = rows <= 512 ? 16 : rows <= 1024 ? 8 : 4;
+    for (int p = pb; p >= 4; p /= 2)
+        if (qr_lds_elems(rows, p) <= lds_elems) return p;
     return 0;
 }
 
@@ -207,7 +232,7 @@ static int clamp_grid(int64_t work) {
 template <typename T>
 hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb) {
     const int64_t m = d.m, n = d.n;
-    const int64_t mu = m < K ? m : K, kk = n < mu ? n : mu;
+    const int64_t mu = m < K ? m : K;
     hipLaunchKernelGGL(k_stack<T>, dim3(clamp_grid(m * K)), dim3(NT), 0, s, d);
     hipError_t e = launch_qr<T>(s, d, lds_bytes, pb, 0);
     if (e != hipSuccess) return e;
@@ -215,7 +240,6 @@ hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, i
     hipLaunchKernelGGL(k_y<T>, dim3(ytiles), dim3(NT), 0, s, d);
     e = launch_qr<T>(s, d, lds_bytes, pb, 1);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_jmat<T>, dim3(clamp_grid(mu * kk + kk * kk)), dim3(NT), 0, s, d);
     hipLaunchKernelGGL(k_svd<T>, dim3(1), dim3(NT), 0, s, d);
     return hipGetLastError();
 }

@@ -29,6 +29,12 @@
 #include "gemm.hpp"
 #include "jacobi.hpp"
 
+// Register-heavy phase variants (Householder panel application, block Jacobi) are
+// compiled out of line by default so that each variant gets its own register allocation.
+#ifndef HCAMD_PHASE_ATTR
+#define HCAMD_PHASE_ATTR __noinline__
+#endif
+
 namespace hcamd {
 
 // Header slots at the start of each update's workspace (int64).
@@ -38,7 +44,9 @@ enum : int {
     HDR_SIZES = 2,   // size_s = min(m, n, K)
     HDR_SWEEPS = 3,  // Jacobi sweeps used
     HDR_STATUS = 4,  // status bits (STATUS_*)
-    HDR_WORDS = 8,
+    HDR_KT = 5,      // columns of the truncated core kept for the SVD (<= size_s)
+    HDR_T0 = 8,      // 8 phase timestamps (builds with HCAMD_PHASE_TIMING only)
+    HDR_WORDS = 16,
 };
 enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2, STATUS_TOO_BIG = 4 };
 
@@ -66,7 +74,7 @@ struct RecompDesc {
 
 // Workspace layout (element offsets of T) for capacity kmax and tile m x n.
 struct WsLayout {
-    int64_t hdr, us, vs, y, tau_u, tau_y, t_u, t_y, j, v1, sig, perm, vtmp, total;
+    int64_t hdr, us, vs, y, tau_u, tau_y, t_u, t_y, j, v1, sig, perm, piv, vtmp, total;
 };
 
 __host__ __device__ inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
@@ -90,6 +98,7 @@ __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax
     L.v1 = o; o += align_up(kk * kk, 32);
     L.sig = o; o += align_up(kk, 32);
     L.perm = o; o += align_up((kk * 4 + sizeof(T) - 1) / sizeof(T), 32);
+    L.piv = o; o += align_up((kmax * 4 + sizeof(T) - 1) / sizeof(T), 32);
     L.vtmp = o; o += align_up(n * (max_rank > 0 ? max_rank : 1), 32);
     L.total = o;
     return L;
@@ -100,6 +109,7 @@ struct Ws {
     int64_t *hdr;
     T *us, *vs, *y, *tau_u, *tau_y, *t_u, *t_y, *j, *v1, *sig, *vtmp;
     int32_t *perm;
+    int32_t *piv;  // column permutation of the pivoted QR of Y
 };
 
 template <typename T>
@@ -118,12 +128,14 @@ __device__ inline Ws<T> ws_of(const RecompDesc<T> &d) {
     w.v1 = d.ws + L.v1;
     w.sig = d.ws + L.sig;
     w.perm = reinterpret_cast<int32_t *>(d.ws + L.perm);
+    w.piv = reinterpret_cast<int32_t *>(d.ws + L.piv);
     w.vtmp = d.ws + L.vtmp;
     return w;
 }
 
 struct Dims {
     int K, mu, kk;
+    int kt;  // truncated core width (valid after ph_core)
 };
 
 template <typename T>
@@ -132,6 +144,7 @@ __device__ inline Dims dims_of(const RecompDesc<T> &d, const Ws<T> &w) {
     z.K = (int) w.hdr[HDR_K];
     z.mu = d.m < z.K ? d.m : z.K;
     z.kk = d.n < z.mu ? d.n : z.mu;
+    z.kt = (int) w.hdr[HDR_KT];
     return z;
 }
 
@@ -205,7 +218,7 @@ __device__ void apply_wy(const T *P, const T *Tm, int rows, T *B, int64_t ldb, i
                 part[c * PB + k] = s;
             }
         }
-        const T tot = wave_multi_sum<T, NCB * PB>(part);  // lane l holds index l
+        const T tot = wave_multi_sum_fast<T, NCB * PB>(part);  // lane l holds index l
         red[wv * WAVE + l] = tot;
         __syncthreads();
         if (threadIdx.x < WAVE) {
@@ -254,7 +267,7 @@ __device__ void apply_wy(const T *P, const T *Tm, int rows, T *B, int64_t ldb, i
 // gets its own register allocation (inlining them all into one kernel makes
 // the widest variant's pressure spill the others).
 template <typename T, int PB, int RPT>
-__device__ __noinline__ void apply_wy_nl(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
+__device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
                                           bool trans, T *red) {
     apply_wy<T, PB, RPT>(P, Tm, rows, B, ldb, c0, c1, trans, red);
 }
@@ -262,11 +275,14 @@ __device__ __noinline__ void apply_wy_nl(const T *P, const T *Tm, int rows, T *B
 template <typename T, int PB>
 __device__ void apply_wy_dispatch(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1, bool trans,
                                   T *red) {
+    // rows <= 512 for PB 16, <= 1024 for PB 8, <= 2048 for PB 4 (pick_pb): at most 32 panel values per lane
     const int chunks = (rows + WAVE * NWAVES - 1) / (WAVE * NWAVES);
     if (chunks <= 1) apply_wy_nl<T, PB, 1>(P, Tm, rows, B, ldb, c0, c1, trans, red);
     else if (chunks <= 2) apply_wy_nl<T, PB, 2>(P, Tm, rows, B, ldb, c0, c1, trans, red);
-    else if (chunks <= 4) apply_wy_nl<T, PB, 4>(P, Tm, rows, B, ldb, c0, c1, trans, red);
-    else if constexpr (PB <= 8) apply_wy_nl<T, PB, 8>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+    else if constexpr (PB <= 8) {
+        if (chunks <= 4) apply_wy_nl<T, PB, 4>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+        else if constexpr (PB <= 4) apply_wy_nl<T, PB, 8>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+    }
 }
 
 // Build T (PB x PB, upper, column-major in LDS) for the panel P with taus tau[0..jb):
@@ -303,21 +319,102 @@ __host__ __device__ inline int64_t qr_lds_elems(int64_t m, int pb) {
     return m * pb + 2 * pb * pb + (NWAVES + 2) * WAVE + 64;
 }
 
+// Window pivoting of a blocked QR at panel start j0: the jb trailing columns of largest
+// norm over rows [j0, m) are moved, in decreasing norm order, to positions [j0, j0 + jb)
+// by jb column swaps (whole columns, so the R rows above j0 move with them); piv records
+// the permutation.  Uses the panel buffer P as scratch (needs m * PB >= 2 * (ncols - j0) + 2 * PB).
+template <typename T, int PB>
+__device__ void window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, int32_t *piv, T *P) {
+    const int nt = ncols - j0;
+    const int l = lane_id();
+    T *nrm = P;
+    int *at = reinterpret_cast<int *>(nrm + nt);  // at[p]: column (by starting position) now at p
+    int *where = at + nt;                         // where[c]: current position of column c
+    int *sel = where + nt;                        // sel[t]: t-th largest column; sel[PB + t]: swap source
+    for (int c = wave_id(); c < nt; c += NWAVES) {
+        const T *col = A + (int64_t) (j0 + c) * lda;
+        T sum = T(0);
+        for (int r = j0 + l; r < m; r += WAVE) {
+            const T v = col[r];
+            sum += v * v;
+        }
+        sum = wave_sum(sum);
+        if (l == 0) nrm[c] = sum;
+    }
+    for (int c = threadIdx.x; c < nt; c += NT) {
+        at[c] = c;
+        where[c] = c;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nt; c += NT) {
+        const T v = nrm[c];
+        int pos = 0;
+        for (int k = 0; k < nt; ++k) {
+            const T u = nrm[k];
+            pos += (u > v) || (u == v && k < c);
+        }
+        if (pos < jb) sel[pos] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int t = 0; t < jb; ++t) {
+            const int c = sel[t], p = where[c];
+            sel[PB + t] = p;
+            if (p != t) {
+                const int c2 = at[t];
+                at[t] = c;
+                at[p] = c2;
+                where[c] = t;
+                where[c2] = p;
+                const int32_t x = piv[j0 + t];
+                piv[j0 + t] = piv[j0 + p];
+                piv[j0 + p] = x;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = 0; t < jb; ++t) {
+        const int p = sel[PB + t];
+        if (p != t) {
+            T *ca = A + (int64_t) (j0 + t) * lda;
+            T *cb = A + (int64_t) (j0 + p) * lda;
+            for (int r = threadIdx.x; r < m; r += NT) {
+                const T x = ca[r];
+                ca[r] = cb[r];
+                cb[r] = x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Whether window pivoting fits the panel buffer of a QR of m x ncols with panel PB.
+__host__ __device__ inline bool window_pivot_fits(int64_t m, int64_t ncols, int pb) {
+    return m * pb >= 2 * ncols + 2 * pb + 8;
+}
+
 // ---------------------------------------------------------------------------
 // Blocked Householder QR (LAPACK dgeqrf convention) of A (m x ncols, ld lda) in
 // place; tau[0..min(m,ncols)); the T factor of panel p is saved to tsave[p*PB*PB].
+// With piv non-null, each panel starts with window pivoting (window_pivot) and
+// piv[0..ncols) receives the column permutation: A P = Q R.
 // ---------------------------------------------------------------------------
 template <typename T, int PB>
-__device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds) {
+__device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr) {
     T *P = lds;
     T *Tm = P + (int64_t) m * PB;
     T *gram = Tm + PB * PB;
     T *red = gram + PB * PB;  // (NWAVES + 2) * WAVE
     const int nsteps = m < ncols ? m : ncols;
     const int l = lane_id();
+    if (piv) {
+        for (int c = threadIdx.x; c < ncols; c += NT) piv[c] = c;
+        __syncthreads();
+    }
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
         const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
         const int rows = m - j0;
+        if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P);
         for (int e = threadIdx.x; e < rows * PB; e += NT) {
             const int r = e % rows, c = e / rows;
             P[e] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
@@ -361,7 +458,7 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
                     for (int k = 0; k < PB; ++k)
                         if (k > jj && k < jb) acc[k] += v * P[r + k * rows];
                 }
-                const T s = wave_multi_sum<T, PB>(acc);
+                const T s = wave_multi_sum_fast<T, PB>(acc);
                 const int idx = wave_multi_sum_index<PB>(l);
                 if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
                 __syncthreads();
@@ -436,6 +533,20 @@ __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *ts
     }
 }
 
+// Phase: window-pivoted QR of Y (n x mu); piv is the identity when the panel buffer is too
+// small for pivoting (very wide Y).
+template <typename T, int PB>
+__device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const bool pivot = window_pivot_fits(d.n, z.mu, PB);
+    if (!pivot) {
+        for (int c = threadIdx.x; c < z.mu; c += NT) w.piv[c] = c;
+        __syncthreads();
+    }
+    wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds, pivot ? w.piv : nullptr);
+}
+
 // ---------------------------------------------------------------------------
 // Phase: Y = V_s R_U^T  (n x mu),  R_U = triu(U_s[0:mu, 0:K]).
 // Output tiles are dealt to workgroups `wg` of `nwg`.
@@ -450,59 +561,111 @@ __device__ void ph_y(const RecompDesc<T> &d, int wg, int nwg, T *lds) {
                         lds, /*upper_b=*/true);
 }
 
-// Phase: J = R_Y^T (mu x kk, ld mu) and V1 = I (kk x kk).
+// Phase (single workgroup): truncate the core and set up the Jacobi SVD.
+// R_Y (kk x mu, upper, in the leading rows of Y) gives the core J = R_Y^T (mu x kk).  Rows
+// j >= kt of R_Y whose total Frobenius norm is at most eps * ||R_Y||_F are dropped: the
+// singular values of J move by at most that much, i.e. by the rounding level of the
+// factorisation itself.  No truncation when the requested threshold is within a small factor
+// of that level, and never below a requested fixed rank.  Writes kt to the header, builds
+// J = R_Y[0:kt, :]^T (mu x kt, ld mu) and V1 = I (kt x kt).  `lds` holds >= kk + NWAVES values.
 template <typename T>
-__device__ void ph_jmat(const RecompDesc<T> &d, int wg, int nwg) {
+__device__ void ph_core(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
-    const int64_t stride = (int64_t) nwg * NT;
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < (int64_t) z.mu * z.kk; e += stride) {
-        const int64_t i = e % z.mu, j = e / z.mu;
-        w.j[e] = (i >= j) ? w.y[j + i * d.n] : T(0);
+    const int mu = z.mu, kk = z.kk, n = d.n, l = lane_id();
+    T *rn = lds;         // rn[j] = ||R_Y[j, j:mu]||^2, then suffix sums
+    T *red = lds + kk;   // NWAVES
+    for (int j = wave_id(); j < kk; j += NWAVES) {
+        T sum = T(0);
+        for (int c = j + l; c < mu; c += WAVE) {
+            const T v = w.y[j + (int64_t) c * n];
+            sum += v * v;
+        }
+        sum = wave_sum(sum);
+        if (l == 0) rn[j] = sum;
     }
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < (int64_t) z.kk * z.kk; e += stride) {
-        const int64_t i = e % z.kk, j = e / z.kk;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = T(0);
+        for (int j = kk - 1; j >= 0; --j) {
+            acc += rn[j];
+            rn[j] = acc;  // ||R_Y[j:, :]||_F^2
+        }
+        const T fro2 = kk > 0 ? rn[0] : T(0);
+        const T delta = Eps<T>::value * sqrt(fro2);
+        int kt = kk;
+        // threshold the rank rule will use, bounded below (sigma_0 >= ||J||_F / sqrt(kk))
+        const T thr_lo = d.truncated_svd ? T(d.accuracy) * sqrt(fro2 / T(kk > 0 ? kk : 1)) : T(d.accuracy);
+        if (thr_lo > T(64) * delta) {
+            for (int j = 1; j < kk; ++j) {
+                if (sqrt(rn[j]) <= delta) {
+                    kt = j;
+                    break;
+                }
+            }
+        }
+        if (d.fixed_rank > 0) {
+            const int fr = d.fixed_rank < kk ? d.fixed_rank : kk;
+            if (kt < fr) kt = fr;
+        }
+        if (kt < 1) kt = kk < 1 ? 0 : 1;
+        w.hdr[HDR_KT] = kt;
+        red[0] = T(kt);
+    }
+    __syncthreads();
+    const int kt = (int) red[0];
+    for (int64_t e = threadIdx.x; e < (int64_t) mu * kt; e += NT) {
+        const int64_t i = e % mu, j = e / mu;
+        w.j[e] = (i >= j) ? w.y[j + i * n] : T(0);
+    }
+    for (int64_t e = threadIdx.x; e < (int64_t) kt * kt; e += NT) {
+        const int64_t i = e % kt, j = e / kt;
         w.v1[e] = (i == j) ? T(1) : T(0);
     }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // Phase: Jacobi + sigma + sort + rank (single workgroup).
 // ---------------------------------------------------------------------------
 template <typename T, int RPL, int TW>
-__device__ __noinline__ int block_jacobi_nl(T *J, int rows, int ncols, T *V, T *gbuf) {
+__device__ HCAMD_PHASE_ATTR int block_jacobi_nl(T *J, int rows, int ncols, T *V, T *gbuf) {
     return wg_block_jacobi<T, RPL, TW>(J, rows, ncols, rows, V, ncols, JACOBI_MAX_SWEEPS, gbuf);
 }
 
 template <typename T>
 __device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
+    // (rows per lane, tile width) pairs that fit 256 VGPRs without spilling (two waves per SIMD)
     if (rows <= 64) return block_jacobi_nl<T, 1, 16>(J, rows, ncols, V, gbuf);
-    if (rows <= 128) return block_jacobi_nl<T, 2, 16>(J, rows, ncols, V, gbuf);
-    if (rows <= 256) return block_jacobi_nl<T, 4, 16>(J, rows, ncols, V, gbuf);
-    if (rows <= 384) return block_jacobi_nl<T, 6, 8>(J, rows, ncols, V, gbuf);
-    if (rows <= 512) return block_jacobi_nl<T, 8, 8>(J, rows, ncols, V, gbuf);
-    return block_jacobi_nl<T, 12, 8>(J, rows, ncols, V, gbuf);
+    if (rows <= 128) return block_jacobi_nl<T, 2, 8>(J, rows, ncols, V, gbuf);
+    if (rows <= 192) return block_jacobi_nl<T, 3, 8>(J, rows, ncols, V, gbuf);
+    if (rows <= 256) return block_jacobi_nl<T, 4, 8>(J, rows, ncols, V, gbuf);
+    if (rows <= 384) return block_jacobi_nl<T, 6, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 512) return block_jacobi_nl<T, 8, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 640) return block_jacobi_nl<T, 10, 4>(J, rows, ncols, V, gbuf);
+    return block_jacobi_nl<T, 12, 4>(J, rows, ncols, V, gbuf);
 }
 
 constexpr int JACOBI_ROWS_MAX = 768;
 constexpr int JACOBI_LDS_ELEMS = block_jacobi_lds_elems<double, 16>() + 8;
+static_assert(JACOBI_LDS_ELEMS >= JACOBI_ROWS_MAX + NWAVES, "ph_core scratch must fit the core-SVD LDS");
 
 template <typename T>
 __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
-    const int mu = z.mu, kk = z.kk, K = z.K;
+    const int mu = z.mu, kk = z.kk, kt = z.kt, K = z.K;
     __shared__ int s_nan;
     int sweeps = 0;
     if (mu > JACOBI_ROWS_MAX) {
         if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
     } else {
-        sweeps = run_block_jacobi<T>(w.j, mu, kk, w.v1, lds);
+        sweeps = run_block_jacobi<T>(w.j, mu, kt, w.v1, lds);
     }
     if (threadIdx.x == 0) s_nan = 0;
     __syncthreads();
     const int l = lane_id();
-    for (int j = wave_id(); j < kk; j += NWAVES) {
+    for (int j = wave_id(); j < kt; j += NWAVES) {
         const T *zj = w.j + (int64_t) j * mu;
         T s = 0;
         for (int r = l; r < mu; r += WAVE) s += zj[r] * zj[r];
@@ -513,10 +676,10 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
         }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < kk; j += NT) {
+    for (int j = threadIdx.x; j < kt; j += NT) {
         const T sj = w.sig[j];
         int pos = 0;
-        for (int k = 0; k < kk; ++k) {
+        for (int k = 0; k < kt; ++k) {
             const T sk = w.sig[k];
             pos += (sk > sj) || (sk == sj && k < j);
         }
@@ -531,11 +694,13 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
             if (rk > K) rk = K;
             if (rk > size_s) rk = size_s;
         } else {
+            // singular values beyond the truncated core width kt are below eps * ||J||_F: zero
             rk = size_s;
-            const T s0 = size_s > 0 ? w.sig[w.perm[0]] : T(0);
+            const T s0 = kt > 0 ? w.sig[w.perm[0]] : T(0);
             const T thr = d.truncated_svd ? T(d.accuracy) * s0 : T(d.accuracy);
             for (int i = 1; i < size_s; ++i) {
-                if (w.sig[w.perm[i]] < thr) {
+                const T si = i < kt ? w.sig[w.perm[i]] : T(0);
+                if (si < thr) {
                     rk = i;
                     break;
                 }
@@ -544,6 +709,7 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
         if (rk < 1) rk = 1;
         if (rk > d.max_rank) rk = d.max_rank;
         if (rk > size_s && size_s > 0) rk = size_s;
+        if (rk > kt && kt > 0) rk = kt;
         w.hdr[HDR_RANK] = rk;
         w.hdr[HDR_SIZES] = size_s;
         w.hdr[HDR_SWEEPS] = sweeps;
@@ -552,7 +718,7 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
         if (s_nan) st |= STATUS_NAN;
         w.hdr[HDR_STATUS] |= st;
         if (d.sigma_out) {
-            for (int i = 0; i < size_s; ++i) d.sigma_out[i] = (double) w.sig[w.perm[i]];
+            for (int i = 0; i < size_s; ++i) d.sigma_out[i] = i < kt ? (double) w.sig[w.perm[i]] : 0.0;
         }
     }
     __syncthreads();
@@ -560,7 +726,7 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
 
 // ---------------------------------------------------------------------------
 // Phase: reconstruction of columns [c0, c1) of the new rank.
-//   side 0: C_U[:, c] = Q_U [z_perm(c) / sigma; 0]
+//   side 0: C_U[:, c] = Q_U [P z_perm(c) / sigma; 0]   (P: column pivoting of the QR of Y)
 //   side 1: vtmp[:, c] = Q_Y [v1_perm(c) * sigma; 0]
 // ---------------------------------------------------------------------------
 template <typename T, int PB>
@@ -570,18 +736,23 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
     const int r = (int) w.hdr[HDR_RANK];
     if (c1 > r) c1 = r;
     if (c0 >= c1) return;
-    const int m = d.m, n = d.n, mu = z.mu, kk = z.kk;
+    const int m = d.m, n = d.n, mu = z.mu, kk = z.kk, kt = z.kt;
     if (side == 0) {
         T *B = d.cdata;  // ld m
+        // row i < mu of the core's left factor belongs to row piv[i] of Q_U's basis
         for (int64_t e = threadIdx.x; e < (int64_t) m * (c1 - c0); e += NT) {
             const int i = (int) (e % m), c = c0 + (int) (e / m);
             T v = T(0);
-            if (i < mu && c < kk) {
-                const int j = w.perm[c];
-                const T sg = w.sig[j];
-                v = (sg > T(0)) ? w.j[i + (int64_t) j * mu] / sg : T(0);
+            int row = i;
+            if (i < mu) {
+                row = w.piv[i];
+                if (c < kt) {
+                    const int j = w.perm[c];
+                    const T sg = w.sig[j];
+                    v = (sg > T(0)) ? w.j[i + (int64_t) j * mu] / sg : T(0);
+                }
             }
-            B[i + (int64_t) c * m] = v;
+            B[row + (int64_t) c * m] = v;
         }
         __syncthreads();
         wg_apply_q<T, PB>(m, mu, w.us, m, w.t_u, B, m, c0, c1, lds);
@@ -590,9 +761,9 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
         for (int64_t e = threadIdx.x; e < (int64_t) n * (c1 - c0); e += NT) {
             const int i = (int) (e % n), c = c0 + (int) (e / n);
             T v = T(0);
-            if (i < kk && c < kk) {
+            if (i < kt && c < kt) {
                 const int j = w.perm[c];
-                v = w.v1[i + (int64_t) j * kk] * w.sig[j];
+                v = w.v1[i + (int64_t) j * kt] * w.sig[j];
             }
             B[i + (int64_t) c * n] = v;
         }

@@ -165,6 +165,16 @@ hcorepp_status hcorepp_batch_ranks(hcorepp_batch *b, int64_t *ranks_out);
 hcorepp_status hcorepp_batch_set_ranks(hcorepp_batch *b, const int64_t *ranks);
 /* Worst status over the last pass (synchronises): 0, or HCOREPP_ERR_NUMERIC. Also max Jacobi sweeps. */
 hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps);
+/* Launch shape of the persistent kernel (no counterpart in the reference): number of
+ * workgroups pulling updates from the queue (default 2 per CU, at most n).  Setting it drops a
+ * captured graph; capture again afterwards. */
+hcorepp_status hcorepp_batch_set_grid(hcorepp_batch *b, int workgroups);
+int hcorepp_batch_get_grid(hcorepp_batch *b);
+/* Diagnostics (no counterpart in the reference): per-update phase timestamps of the last pass,
+ * 8 int64 per update in ticks of the 100 MHz device wall clock (start, after the factor products,
+ * after QR(U_s), after Y, after QR(Y), after the core SVD, after U, end).  Returns
+ * HCOREPP_ERR_UNSUPPORTED unless the library was built with -DHCAMD_PHASE_TIMING. */
+hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times);
 /* Bytes of device workspace the batch holds. */
 size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b);
 hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b);
