@@ -446,9 +446,9 @@ class Batch:
         return lib().hcorepp_batch_get_grid(self.h)
 
     def phase_times(self) -> np.ndarray:
-        """Per-update phase timestamps of the last pass (n x 8, 100 MHz ticks); needs a
-        library built with HCAMD_PHASE_TIMING."""
-        out = np.zeros((self.n, 8), dtype=np.int64)
+        """Per-update phase timestamps and sub-phase durations of the last pass (n x 16,
+        100 MHz ticks); needs a library built with HCAMD_PHASE_TIMING."""
+        out = np.zeros((self.n, 16), dtype=np.int64)
         _check(lib().hcorepp_batch_phase_times(self.h, out.ctypes.data_as(C.c_void_p)))
         return out
 

@@ -735,8 +735,8 @@ hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times) {
 #else
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     for (int64_t i = 0; i < b->n; ++i)
-        HIP_TRY(hipMemcpy(times + 8 * i, b->ws + b->hdr_off[(size_t) i] + HDR_T0 * sizeof(int64_t),
-                          8 * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(times + 16 * i, b->ws + b->hdr_off[(size_t) i] + HDR_T0 * sizeof(int64_t),
+                          16 * sizeof(int64_t), hipMemcpyDeviceToHost));
     return HCOREPP_OK;
 #endif
 }

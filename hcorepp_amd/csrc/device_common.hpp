@@ -27,12 +27,6 @@ struct Eps<float> {
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
-}
 
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
@@ -43,6 +37,10 @@ __device__ __forceinline__ T wave_max(T v) {
     }
     return v;
 }
+
+// Wave-wide sum (every lane gets the total); needs the whole wave active.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v);
 
 // Workgroup-wide sum.  `red` is an LDS scratch of at least NWAVES elements.
 // All threads of the workgroup must call it; every thread gets the total.
@@ -212,6 +210,12 @@ __device__ __forceinline__ T wave_multi_sum_fast(T (&v)[NV]) {
     static_assert((NV & (NV - 1)) == 0 && NV <= 64, "NV must be a power of two <= 64");
     MultiSumFast<T, NV, NV, 5>::run(v, lane_id());
     return v[0];
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    T a[1] = {v};
+    return wave_multi_sum_fast<T, 1>(a);
 }
 
 // Value of lane `src` (a compile-time or wave-uniform lane index) in every lane.

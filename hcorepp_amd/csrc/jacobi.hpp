@@ -9,10 +9,8 @@
 // multi-value wave reduction.  Block pairs follow the circle (round-robin)
 // ordering; step 0 of every sweep rotates all pairs inside the tile (so the
 // intra-block pairs of every block are visited once per sweep), later steps
-// only the cross pairs.  Rotations of a tile are accumulated in a TW x TW
-// matrix G whose row l lives in lane l's registers (every lane applies the
-// broadcast rotation to its own row); G goes through LDS once per tile and is
-// applied to V's block columns afterwards.
+// only the cross pairs.  The V columns of a tile are loaded with its J columns
+// and rotated alongside them, so a tile costs one memory round trip.
 //
 // Convergence test: |z_p . z_q| <= sqrt(rows) * eps * |z_p| |z_q| (relative,
 // as LAPACK's one-sided Jacobi), pairs involving a column whose norm is below
@@ -76,10 +74,11 @@ __device__ __forceinline__ void jacobi_rotation(T app, T aqq, T apq, T tol, T fl
 // One round: TW/2 disjoint pairs of the register tile y.  The 3*TW/2 dot products are
 // reduce-scattered across the wave with permlane/DPP exchanges; lane k < TW/2 gathers the three
 // sums of pair k, computes its rotation, and every lane reads the rotations back with readlane.
-template <typename T, int RPL, int TW, bool FULL>
-__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&g)[TW], int rd, T tol, T floor2) {
+// The rotations are applied to the J tile y and to the matching V columns vt.
+template <typename T, int RPL, int RPLV, int TW, bool FULL>
+__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], int rd, T tol, T floor2) {
     constexpr int NP = TW / 2;
-    constexpr int NV = (3 * NP <= 16) ? 16 : 32;
+    constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
     const int l = lane_id();
     T part[NV];
 #pragma unroll
@@ -115,10 +114,12 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&g)[TW], int rd,
                 y[r][p] = ck * a - sk * b;
                 y[r][q] = sk * a + ck * b;
             }
-            // lane-local row of G <- G * R_pq (lane l < TW owns row l of G)
-            const T gp = g[p], gq = g[q];
-            g[p] = ck * gp - sk * gq;
-            g[q] = sk * gp + ck * gq;
+#pragma unroll
+            for (int r = 0; r < RPLV; ++r) {
+                const T va = vt[r][p], vb = vt[r][q];
+                vt[r][p] = ck * va - sk * vb;
+                vt[r][q] = sk * va + ck * vb;
+            }
             any = 1;
         }
     }
@@ -126,13 +127,15 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&g)[TW], int rd,
 }
 
 // Process one tile (block pair) with one wave.  Columns [cI, cI+TW/2) and [cJ, cJ+TW/2)
-// of J (rows x ncols, ld ldj); columns >= ncols are zero and never stored.
-template <typename T, int RPL, int TW>
+// of J (rows x ncols, ld ldj; RPL rows per lane) and of V (ncols x ncols, ld ldv; RPLV rows
+// per lane) are loaded together, so the tile costs one memory round trip; columns >= ncols
+// are zero and never stored.
+template <typename T, int RPL, int RPLV, int TW>
 __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ, bool full,
-                           T tol, T floor2, T *G) {
+                           T tol, T floor2) {
     constexpr int B = TW / 2;
     const int l = lane_id();
-    T y[RPL][TW];
+    T y[RPL][TW], vt[RPLV][TW];
 #pragma unroll
     for (int c = 0; c < TW; ++c) {
         const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -141,28 +144,21 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
             const int r = l + WAVE * t;
             y[t][c] = (r < rows && gc < ncols) ? J[r + (int64_t) gc * ldj] : T(0);
         }
-    }
-    T g[TW];
 #pragma unroll
-    for (int c = 0; c < TW; ++c) g[c] = (c == l) ? T(1) : T(0);
+        for (int t = 0; t < RPLV; ++t) {
+            const int r = l + WAVE * t;
+            vt[t][c] = (r < ncols && gc < ncols) ? V[r + (int64_t) gc * ldv] : T(0);
+        }
+    }
     int any = 0;
     if (full) {
 #pragma unroll
-        for (int rd = 0; rd < TW - 1; ++rd) any |= jacobi_round<T, RPL, TW, true>(y, g, rd, tol, floor2);
+        for (int rd = 0; rd < TW - 1; ++rd) any |= jacobi_round<T, RPL, RPLV, TW, true>(y, vt, rd, tol, floor2);
     } else {
 #pragma unroll
-        for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, TW, false>(y, g, rd, tol, floor2);
+        for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, RPLV, TW, false>(y, vt, rd, tol, floor2);
     }
     if (any) {
-        // publish G (row l from lane l) to the wave's LDS slot; wavefront-scope fences order the
-        // cross-lane exchange
-        if (l < TW) {
-#pragma unroll
-            for (int c = 0; c < TW; ++c) G[l + TW * c] = g[c];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int c = 0; c < TW; ++c) {
             const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -171,45 +167,31 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
                 const int r = l + WAVE * t;
                 if (r < rows && gc < ncols) J[r + (int64_t) gc * ldj] = y[t][c];
             }
-        }
-        __builtin_amdgcn_wave_barrier();
-        // V[:, tile] <- V[:, tile] * G   (V is ncols x ncols)
-        for (int r = l; r < ncols; r += WAVE) {
-            // re-read G from LDS every row batch instead of holding its TW*TW values in registers
-            __asm__ volatile("" ::: "memory");
-            T v[TW];
 #pragma unroll
-            for (int c = 0; c < TW; ++c) {
-                const int gc = (c < B) ? cI + c : cJ + (c - B);
-                v[c] = (gc < ncols) ? V[r + (int64_t) gc * ldv] : T(0);
-            }
-#pragma unroll
-            for (int c2 = 0; c2 < TW; ++c2) {
-                const int gc = (c2 < B) ? cI + c2 : cJ + (c2 - B);
-                T s = T(0);
-#pragma unroll
-                for (int c = 0; c < TW; ++c) s += v[c] * G[c + TW * c2];
-                if (gc < ncols) V[r + (int64_t) gc * ldv] = s;
+            for (int t = 0; t < RPLV; ++t) {
+                const int r = l + WAVE * t;
+                if (r < ncols && gc < ncols) V[r + (int64_t) gc * ldv] = vt[t][c];
             }
         }
     }
     return any;
 }
 
-// LDS elements wg_block_jacobi needs: per-wave G matrices, a reduction slot per wave, a flag.
+// LDS elements wg_block_jacobi needs: a reduction slot per wave and a flag.
 template <typename T, int TW>
 __host__ __device__ constexpr int block_jacobi_lds_elems() {
-    return NWAVES * TW * TW + NWAVES + 2;
+    return NWAVES + 2;
 }
 
-// Whole-workgroup block Jacobi.  J: rows x ncols (ld ldj), V: ncols x ncols (ld ldv) must hold
-// the identity on entry.  `gbuf` is LDS of block_jacobi_lds_elems<T, TW>() elements (all the
-// LDS this routine touches, so it may be called out of line).  Returns the sweeps used
-// (max_sweeps when it did not converge).
-template <typename T, int RPL, int TW>
+// Whole-workgroup block Jacobi.  J: rows x ncols (ld ldj, rows <= 64 RPL), V: ncols x ncols
+// (ld ldv, ncols <= 64 RPLV) must hold the identity on entry.
+// `gbuf` is LDS of block_jacobi_lds_elems<T, TW>() elements (all the LDS this routine touches,
+// so it may be called out of line).  Returns the sweeps used (max_sweeps when it did not
+// converge).
+template <typename T, int RPL, int RPLV, int TW>
 __device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int max_sweeps, T *gbuf) {
     constexpr int B = TW / 2;
-    T *s_red = gbuf + NWAVES * TW * TW;
+    T *s_red = gbuf;
     int &s_rot = *reinterpret_cast<int *>(s_red + NWAVES);
     T part = T(0);
     for (int c = 0; c < ncols; ++c)
@@ -222,7 +204,6 @@ __device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int
     const T tol = sqrt(T(rows)) * Eps<T>::value;
     const int nblk = (ncols + B - 1) / B;
     const int nbp = nblk + (nblk & 1);
-    T *G = gbuf + wave_id() * TW * TW;
     int sweep = 0;
     if (ncols <= 1) return 0;
     for (; sweep < max_sweeps; ++sweep) {
@@ -245,8 +226,8 @@ __device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int
                     bJ = t;
                 }
                 if (bI * B >= ncols) continue;
-                any |= jacobi_tile<T, RPL, TW>(J, rows, ncols, ldj, V, ldv, bI * B, bJ * B, step == 0, tol, floor2,
-                                               G);
+                any |= jacobi_tile<T, RPL, RPLV, TW>(J, rows, ncols, ldj, V, ldv, bI * B, bJ * B, step == 0, tol,
+                                                     floor2);
             }
             __syncthreads();
         }

@@ -117,6 +117,10 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         if (item >= n) break;
         const BatchEntry<T> &e = entries[order[item]];
         const RecompDesc<T> &d = e.rd;
+#ifdef HCAMD_PHASE_TIMING
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 8; ++k) ws_of(d).hdr[HDR_SUB + k] = 0;
+#endif
         const uint64_t t_start = HCAMD_TSTAMP();
         run_temp_gemms<T>(e, lds);
         const uint64_t t_gemm = HCAMD_TSTAMP();
@@ -126,7 +130,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         const Dims z = dims_of(d, w);
         uint64_t t_qru = 0, t_y = 0, t_qry = 0, t_svd = 0, t_ru = 0;
         if (z.K <= d.kmax) {
-            wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds);
+            wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds, nullptr, w.hdr);
             __syncthreads();
             t_qru = HCAMD_TSTAMP();
             ph_y<T>(d, 0, 1, lds);

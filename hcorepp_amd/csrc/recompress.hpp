@@ -37,6 +37,19 @@
 
 namespace hcamd {
 
+// Sub-phase timers of the timing build: HCAMD_SUB_BEGIN(tv) starts stopwatch tv,
+// HCAMD_SUB_END(tv, hdr, slot) adds its elapsed device clock (thread 0) to header word HDR_SUB + slot.
+#ifdef HCAMD_PHASE_TIMING
+#define HCAMD_SUB_BEGIN(tv) const uint64_t tv = (uint64_t) wall_clock64()
+#define HCAMD_SUB_END(tv, hdr, slot)                                                               \
+    do {                                                                                          \
+        if ((hdr) && threadIdx.x == 0) (hdr)[HDR_SUB + (slot)] += (int64_t) ((uint64_t) wall_clock64() - (tv)); \
+    } while (0)
+#else
+#define HCAMD_SUB_BEGIN(tv) (void) 0
+#define HCAMD_SUB_END(tv, hdr, slot) (void) 0
+#endif
+
 // Header slots at the start of each update's workspace (int64).
 enum : int {
     HDR_K = 0,       // stacked rank K = arank + rc
@@ -46,7 +59,8 @@ enum : int {
     HDR_STATUS = 4,  // status bits (STATUS_*)
     HDR_KT = 5,      // columns of the truncated core kept for the SVD (<= size_s)
     HDR_T0 = 8,      // 8 phase timestamps (builds with HCAMD_PHASE_TIMING only)
-    HDR_WORDS = 16,
+    HDR_SUB = 16,    // 8 sub-phase cycle accumulators (builds with HCAMD_PHASE_TIMING only)
+    HDR_WORDS = 24,
 };
 enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2, STATUS_TOO_BIG = 4 };
 
@@ -299,17 +313,26 @@ __device__ void build_t(const T *P, int rows, const T *tau, int jb, T *Tm, T *gr
         if (l == 0) gram[a + b * PB] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < PB * PB; ++i) Tm[i] = T(0);
-        for (int i = 0; i < jb; ++i) {
-            const T ti = tau[i];
-            Tm[i + i * PB] = ti;
-            for (int a = 0; a < i; ++a) {
-                T s = T(0);
-                for (int b = a; b < i; ++b) s += Tm[a + b * PB] * gram[b + i * PB];
-                Tm[a + i * PB] = -ti * s;
+    // Lane a < PB builds row a of T: T[a][i] = -tau_i sum_{b=a}^{i-1} T[a][b] gram[b][i] only
+    // reads row a, so the rows are independent.
+    if (threadIdx.x < PB) {
+        const int a = threadIdx.x;
+        T trow[PB];
+#pragma unroll
+        for (int b = 0; b < PB; ++b) trow[b] = T(0);
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            if (i < jb) {
+                const T ti = tau[i];
+                T acc = T(0);
+#pragma unroll
+                for (int b = 0; b < i; ++b)
+                    if (b >= a) acc += trow[b] * gram[b + i * PB];
+                trow[i] = (a < i) ? -ti * acc : (a == i ? ti : T(0));
             }
         }
+#pragma unroll
+        for (int b = 0; b < PB; ++b) Tm[a + b * PB] = trow[b];
     }
     __syncthreads();
 }
@@ -400,7 +423,9 @@ __host__ __device__ inline bool window_pivot_fits(int64_t m, int64_t ncols, int 
 // piv[0..ncols) receives the column permutation: A P = Q R.
 // ---------------------------------------------------------------------------
 template <typename T, int PB>
-__device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr) {
+__device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr,
+                         int64_t *hdr = nullptr) {
+    (void) hdr;
     T *P = lds;
     T *Tm = P + (int64_t) m * PB;
     T *gram = Tm + PB * PB;
@@ -414,6 +439,7 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
         const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
         const int rows = m - j0;
+        HCAMD_SUB_BEGIN(t_pan);
         if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P);
         for (int e = threadIdx.x; e < rows * PB; e += NT) {
             const int r = e % rows, c = e / rows;
@@ -481,6 +507,7 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
                 __syncthreads();
             }
         }
+        HCAMD_SUB_END(t_pan, hdr, 0);
         // write the factored panel back (R on/above the diagonal, reflectors below)
         for (int e = threadIdx.x; e < rows * jb; e += NT) {
             const int r = e % rows, c = e / rows;
@@ -495,12 +522,16 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
             }
         }
         __syncthreads();
+        HCAMD_SUB_BEGIN(t_bt);
         build_t<T, PB>(P, rows, tau + j0, jb, Tm, gram);
         if (tsave) {
             for (int e = threadIdx.x; e < PB * PB; e += NT) tsave[(int64_t) (j0 / PB) * PB * PB + e] = Tm[e];
         }
+        HCAMD_SUB_END(t_bt, hdr, 1);
+        HCAMD_SUB_BEGIN(t_up);
         if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, rows, A + j0, lda, j0 + jb, ncols, true, red);
         __syncthreads();
+        HCAMD_SUB_END(t_up, hdr, 2);
     }
 }
 
@@ -544,7 +575,7 @@ __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
         for (int c = threadIdx.x; c < z.mu; c += NT) w.piv[c] = c;
         __syncthreads();
     }
-    wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds, pivot ? w.piv : nullptr);
+    wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds, pivot ? w.piv : nullptr, w.hdr);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,26 +659,49 @@ __device__ void ph_core(const RecompDesc<T> &d, T *lds) {
 // ---------------------------------------------------------------------------
 // Phase: Jacobi + sigma + sort + rank (single workgroup).
 // ---------------------------------------------------------------------------
-template <typename T, int RPL, int TW>
+template <typename T, int RPL, int RPLV, int TW>
 __device__ HCAMD_PHASE_ATTR int block_jacobi_nl(T *J, int rows, int ncols, T *V, T *gbuf) {
-    return wg_block_jacobi<T, RPL, TW>(J, rows, ncols, rows, V, ncols, JACOBI_MAX_SWEEPS, gbuf);
+    return wg_block_jacobi<T, RPL, RPLV, TW>(J, rows, ncols, rows, V, ncols, JACOBI_MAX_SWEEPS, gbuf);
 }
 
+// Tile shape by core rows and (truncated) width: J rows per lane, V rows per lane and tile
+// width, chosen so that both register tiles fit 256 VGPRs without spilling.
 template <typename T>
 __device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
-    // (rows per lane, tile width) pairs that fit 256 VGPRs without spilling (two waves per SIMD)
-    if (rows <= 64) return block_jacobi_nl<T, 1, 16>(J, rows, ncols, V, gbuf);
-    if (rows <= 128) return block_jacobi_nl<T, 2, 8>(J, rows, ncols, V, gbuf);
-    if (rows <= 192) return block_jacobi_nl<T, 3, 8>(J, rows, ncols, V, gbuf);
-    if (rows <= 256) return block_jacobi_nl<T, 4, 8>(J, rows, ncols, V, gbuf);
-    if (rows <= 384) return block_jacobi_nl<T, 6, 4>(J, rows, ncols, V, gbuf);
-    if (rows <= 512) return block_jacobi_nl<T, 8, 4>(J, rows, ncols, V, gbuf);
-    if (rows <= 640) return block_jacobi_nl<T, 10, 4>(J, rows, ncols, V, gbuf);
-    return block_jacobi_nl<T, 12, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 64) return block_jacobi_nl<T, 1, 1, 16>(J, rows, ncols, V, gbuf);
+    if (rows <= 128)
+        return ncols <= 64 ? block_jacobi_nl<T, 2, 1, 8>(J, rows, ncols, V, gbuf)
+                           : block_jacobi_nl<T, 2, 2, 8>(J, rows, ncols, V, gbuf);
+    if (rows <= 192)
+        return ncols <= 128 ? block_jacobi_nl<T, 3, 2, 8>(J, rows, ncols, V, gbuf)
+                            : block_jacobi_nl<T, 3, 3, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 256)
+        return ncols <= 192 ? block_jacobi_nl<T, 4, 3, 4>(J, rows, ncols, V, gbuf)
+                            : block_jacobi_nl<T, 4, 4, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 320)
+        return ncols <= 192 ? block_jacobi_nl<T, 5, 3, 4>(J, rows, ncols, V, gbuf)
+                            : block_jacobi_nl<T, 5, 5, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 384) {
+        if (ncols <= 192) return block_jacobi_nl<T, 6, 3, 4>(J, rows, ncols, V, gbuf);
+        if (ncols <= 256) return block_jacobi_nl<T, 6, 4, 4>(J, rows, ncols, V, gbuf);
+        return block_jacobi_nl<T, 6, 6, 4>(J, rows, ncols, V, gbuf);
+    }
+    if (rows <= 512)
+        return ncols <= 256 ? block_jacobi_nl<T, 8, 4, 4>(J, rows, ncols, V, gbuf)
+                            : block_jacobi_nl<T, 8, 8, 2>(J, rows, ncols, V, gbuf);
+    if (rows <= 640)
+        return ncols <= 320 ? block_jacobi_nl<T, 10, 5, 2>(J, rows, ncols, V, gbuf)
+                            : block_jacobi_nl<T, 10, 10, 2>(J, rows, ncols, V, gbuf);
+    return ncols <= 384 ? block_jacobi_nl<T, 12, 6, 2>(J, rows, ncols, V, gbuf)
+                        : block_jacobi_nl<T, 12, 12, 2>(J, rows, ncols, V, gbuf);
 }
 
 constexpr int JACOBI_ROWS_MAX = 768;
-constexpr int JACOBI_LDS_ELEMS = block_jacobi_lds_elems<double, 16>() + 8;
+// LDS of the core-SVD phase: ph_core's row norms (<= JACOBI_ROWS_MAX + NWAVES) and the block Jacobi's
+// reduction slots, which reuse the same space.
+constexpr int JACOBI_LDS_ELEMS =
+        (JACOBI_ROWS_MAX + NWAVES + 8 > block_jacobi_lds_elems<double, 16>() + 8) ? JACOBI_ROWS_MAX + NWAVES + 8
+                                                                                   : block_jacobi_lds_elems<double, 16>() + 8;
 static_assert(JACOBI_LDS_ELEMS >= JACOBI_ROWS_MAX + NWAVES, "ph_core scratch must fit the core-SVD LDS");
 
 template <typename T>
@@ -660,7 +714,9 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
     if (mu > JACOBI_ROWS_MAX) {
         if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
     } else {
+        HCAMD_SUB_BEGIN(t_jac);
         sweeps = run_block_jacobi<T>(w.j, mu, kt, w.v1, lds);
+        HCAMD_SUB_END(t_jac, w.hdr, 3);
     }
     if (threadIdx.x == 0) s_nan = 0;
     __syncthreads();
@@ -755,7 +811,9 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
             B[row + (int64_t) c * m] = v;
         }
         __syncthreads();
+        HCAMD_SUB_BEGIN(t_aq);
         wg_apply_q<T, PB>(m, mu, w.us, m, w.t_u, B, m, c0, c1, lds);
+        HCAMD_SUB_END(t_aq, w.hdr, 4);
     } else {
         T *B = w.vtmp;  // ld n
         for (int64_t e = threadIdx.x; e < (int64_t) n * (c1 - c0); e += NT) {
@@ -768,7 +826,9 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
             B[i + (int64_t) c * n] = v;
         }
         __syncthreads();
+        HCAMD_SUB_BEGIN(t_aq);
         wg_apply_q<T, PB>(n, kk, w.y, n, w.t_y, B, n, c0, c1, lds);
+        HCAMD_SUB_END(t_aq, w.hdr, 4);
     }
 }
 

@@ -171,8 +171,10 @@ hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps);
 hcorepp_status hcorepp_batch_set_grid(hcorepp_batch *b, int workgroups);
 int hcorepp_batch_get_grid(hcorepp_batch *b);
 /* Diagnostics (no counterpart in the reference): per-update phase timestamps of the last pass,
- * 8 int64 per update in ticks of the 100 MHz device wall clock (start, after the factor products,
- * after QR(U_s), after Y, after QR(Y), after the core SVD, after U, end).  Returns
+ * 16 int64 per update in ticks of the 100 MHz device wall clock: 8 timestamps (start, after the
+ * factor products, after QR(U_s), after Y, after QR(Y) and the core set-up, after the core SVD,
+ * after U, end) and 8 accumulated sub-phase durations (Householder panels, T factors, trailing
+ * updates, core SVD sweeps, Q application, 3 spare).  Returns
  * HCOREPP_ERR_UNSUPPORTED unless the library was built with -DHCAMD_PHASE_TIMING. */
 hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times);
 /* Bytes of device workspace the batch holds. */
