@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=N_UPDATES)
     ap.add_argument("--check-sample", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -131,6 +132,49 @@ def main():
     log(f"[bench] batched FP64: {ms_per_step:.1f} ms/step, kernel {kavg:.1f} ms, {value:.0f} updates/s, "
         f"max sweeps {sweeps}")
 
+    # ---- mixed form on the same updates: factor products on FP32 matrix cores (FP32 accumulate)
+    mixed = None
+    if not args.no_mixed:
+        ranks_fp64 = ranks.copy()
+        # timing: rerun the whole batch in mixed form on the arena tiles
+        bm = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, params, H.PREC_MIXED)
+        if use_graph:
+            bm.capture()
+        restore()
+        bm.set_ranks(r0)
+        (bm.launch_graph() if use_graph else bm.run())
+        ctx.sync()
+        mk = []
+        tm0 = time.perf_counter()
+        for _ in range(args.steps):
+            restore()
+            bm.set_ranks(r0)
+            ctx.record(2)
+            (bm.launch_graph() if use_graph else bm.run())
+            ctx.record(3)
+            mk.append(ctx.elapsed_ms(2, 3))
+        ctx.sync()
+        tm = (time.perf_counter() - tm0) / args.steps
+        ranks_mixed = bm.ranks()
+        stm, swm = bm.status()
+        # the arena now holds the mixed results: keep the first check_sample for the error check
+        mixed_fac = []
+        for i in range(min(args.check_sample, args.updates)):
+            Cs[i].rank = int(ranks_mixed[i])
+            mixed_fac.append(Cs[i].factors())
+        mixed = dict(ms=tm * 1e3, kernel_ms=float(np.mean(mk)), ups=args.updates / tm, status=stm,
+                     ranks=ranks_mixed, fac=mixed_fac)
+        log(f"[bench] batched mixed: {tm * 1e3:.1f} ms/step, kernel {np.mean(mk):.1f} ms, "
+            f"{args.updates / tm:.0f} updates/s, status {stm}")
+        bm.close()
+        # restore the FP64 results for the checks below
+        restore()
+        batch.set_ranks(r0)
+        run_batch()
+        ctx.sync()
+        ranks = batch.ranks()
+        assert np.array_equal(ranks, ranks_fp64)
+
     flops = float(np.sum([alg_flops(NB, a, b, c, r) for a, b, c, r in zip(ra, rb, rc, ranks)]))
     traffic_alg = float(np.sum([alg_bytes(NB, a, b, c, r) for a, b, c, r in zip(ra, rb, rc, ranks)]))
     achieved_tflops = flops / (kavg * 1e-3) / 1e12
@@ -184,6 +228,20 @@ def main():
         errs_dense.append(np.linalg.norm(g - dense) / np.linalg.norm(dense))
     log(f"[bench] rel err vs FP64 oracle (recompressed, first {nchk}): max {max(errs):.2e}; vs dense FP64: max "
         f"{max(errs_dense):.2e}; rank mismatches {rank_mis}/{ncpu}")
+    mixed_out = None
+    if mixed is not None:
+        merrs = []
+        for i in range(min(nchk, len(mixed["fac"]))):
+            u_, v_ = mixed["fac"][i]
+            merrs.append(np.linalg.norm(u_ @ v_ - oC[i].to_dense()) / np.linalg.norm(oC[i].to_dense()))
+        mis = int(np.sum(np.asarray(mixed["ranks"][:ncpu]) != np.asarray(orc_ranks)))
+        log(f"[bench] mixed: rel err vs FP64 oracle max {max(merrs):.2e} (first {len(merrs)}); rank mismatches "
+            f"{mis}/{ncpu}")
+        mixed_out = {"updates_per_s": round(mixed["ups"], 2), "ms_per_step": round(mixed["ms"], 3),
+                     "kernel_ms": round(mixed["kernel_ms"], 3), "rel_err_vs_fp64_oracle_max": float(max(merrs)),
+                     "rank_mismatches_vs_oracle": f"{mis}/{ncpu}",
+                     "meets_1e-8": bool(max(merrs) <= 1e-8),
+                     "form": "batched, factor products on FP32 MFMA with FP32 accumulation; QR/SVD/truncation FP64"}
 
     out = {
         "metric": "TLR tile GEMM+recompression updates/s at rel err <=1e-8 (batched vs plain FP64)",
@@ -212,6 +270,7 @@ def main():
         "rank_mismatches_vs_oracle": f"{rank_mis}/{ncpu}",
         "rel_err_checked_updates": nchk,
         "max_jacobi_sweeps": int(sweeps),
+        "mixed_form": mixed_out,
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved_tflops, 4),

@@ -554,8 +554,8 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     *out = nullptr;
     hcorepp_status st;
     if ((st = ensure_lds_limits())) return st;
-    if (precision != HCOREPP_PREC_FP64)
-        return fail(HCOREPP_ERR_UNSUPPORTED, "precision form %d not built into this library", precision);
+    if (precision != HCOREPP_PREC_FP64 && precision != HCOREPP_PREC_MIXED)
+        return fail(HCOREPP_ERR_INVALID, "unknown precision form %d", precision);
     hcorepp_cparams p = params ? *params : hcorepp_cparams{1e-9, 0, 1, 0, HCOREPP_SVD_GESDD, 0};
     using T = double;
     for (int64_t i = 0; i < n; ++i) {
@@ -708,7 +708,7 @@ hcorepp_status hcorepp_batch_run(hcorepp_batch *b) {
     hipStream_t s = b->ctx->stream;
     HIP_TRY(launch_reset_counters(s, b->ctr));
     HIP_TRY(launch_batched<double>(s, (const BatchEntry<double> *) b->entries, b->order, b->n, b->ctr, b->grid,
-                                   kLdsBytes, b->pb));
+                                   kLdsBytes, b->pb, b->precision == HCOREPP_PREC_MIXED ? 1 : 0));
     return HCOREPP_OK;
 }
 

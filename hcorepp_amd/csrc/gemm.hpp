@@ -97,4 +97,92 @@ __device__ void wg_gemm(int M, int N, int Kd, T alpha, const T *A, int64_t lda, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision tile GEMM: C = op(A) op(B) (alpha = 1, beta = 0) for FP64 operands with the
+// products formed on the FP32 matrix cores (v_mfma_f32_16x16x4f32) and accumulated in FP32.
+// A 64 x 64 output tile per workgroup: wave w computes the 32 x 32 quarter at rows 32 (w / 2),
+// columns 32 (w % 2) as 2 x 2 blocks of 16 x 16.  K is staged through LDS as FP32 in chunks of 16.
+// MFMA lane layout (16x16x4): A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k,
+// D[4 (l / 16) + v][l % 16] in element v of lane l.
+// ---------------------------------------------------------------------------
+typedef float hcamd_floatx4 __attribute__((ext_vector_type(4)));
+constexpr int GTM_LDS_FLOATS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
+
+__device__ inline void wg_gemm_tile_mixed(int M, int N, int Kd, const double *A, int64_t lda, bool ta,
+                                          const double *B, int64_t ldb, bool tb, double *C, int64_t ldc, int ti,
+                                          int tj, float *lds) {
+    float *As = lds;                      // As[k][i]
+    float *Bs = lds + GT_K * (GT_M + 1);  // Bs[k][j]
+    const int i0 = ti * GT_M, j0 = tj * GT_N;
+    const int l = lane_id(), w = wave_id();
+    const int wr = 32 * (w / 2), wc = 32 * (w % 2);
+    hcamd_floatx4 acc[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) acc[p][q] = hcamd_floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+        for (int e = threadIdx.x; e < GT_M * GT_K; e += NT) {
+            int i, k;
+            if (ta) {
+                k = e % GT_K;
+                i = e / GT_K;
+            } else {
+                i = e % GT_M;
+                k = e / GT_M;
+            }
+            const int gi = i0 + i, gk = k0 + k;
+            double v = 0.0;
+            if (gi < M && gk < Kd) v = ta ? A[gk + (int64_t) gi * lda] : A[gi + (int64_t) gk * lda];
+            As[k * (GT_M + 1) + i] = (float) v;
+        }
+        for (int e = threadIdx.x; e < GT_N * GT_K; e += NT) {
+            int j, k;
+            if (tb) {
+                j = e % GT_N;
+                k = e / GT_N;
+            } else {
+                k = e % GT_K;
+                j = e / GT_K;
+            }
+            const int gj = j0 + j, gk = k0 + k;
+            double v = 0.0;
+            if (gj < N && gk < Kd) v = tb ? B[gj + (int64_t) gk * ldb] : B[gk + (int64_t) gj * ldb];
+            Bs[k * (GT_N + 1) + j] = (float) v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GT_K; kk += 4) {
+            const int k = kk + l / 16;
+            float a[2], b[2];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) a[p] = As[k * (GT_M + 1) + wr + 16 * p + (l % 16)];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) b[q] = Bs[k * (GT_N + 1) + wc + 16 * q + (l % 16)];
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p], b[q], acc[p][q], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = i0 + wr + 16 * p + 4 * (l / 16) + v;
+                const int gj = j0 + wc + 16 * q + (l % 16);
+                if (gi < M && gj < N) C[gi + (int64_t) gj * ldc] = (double) acc[p][q][v];
+            }
+}
+
+__device__ inline void wg_gemm_mixed(int M, int N, int Kd, const double *A, int64_t lda, bool ta, const double *B,
+                                     int64_t ldb, bool tb, double *C, int64_t ldc, float *lds) {
+    const int tm = (M + GT_M - 1) / GT_M, tn = (N + GT_N - 1) / GT_N;
+    for (int t = 0; t < tm * tn; ++t)
+        wg_gemm_tile_mixed(M, N, Kd, A, lda, ta, B, ldb, tb, C, ldc, t % tm, t / tm, lds);
+}
+
 }  // namespace hcamd

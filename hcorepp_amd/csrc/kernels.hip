@@ -93,10 +93,20 @@ __global__ void HCAMD_KERNEL k_readjust_ddc(const T *W, int64_t m, int64_t n, T 
 // ---------------------------------------------------------------------------
 // Batched persistent kernel.
 // ---------------------------------------------------------------------------
+// Factor products of one update (C.Gemm's temporaries).  mixed: FP32 matrix-core products
+// with FP32 accumulation (FP64 operands and results), otherwise FP64.
 template <typename T>
-__device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds) {
+__device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds, int mixed) {
     for (int g = 0; g < e.ngemm; ++g) {
         const GemmSpec<T> &s = e.gemm[g];
+        if constexpr (sizeof(T) == 8) {
+            if (mixed) {
+                wg_gemm_mixed(s.m, s.n, s.k, s.a, s.lda, s.ta, s.b, s.ldb, s.tb, s.c, s.ldc,
+                              reinterpret_cast<float *>(lds));
+                __syncthreads();
+                continue;
+            }
+        }
         wg_gemm<T>(s.m, s.n, s.k, T(1), s.a, s.lda, s.ta, s.b, s.ldb, s.tb, T(0), s.c, s.ldc, lds);
         __syncthreads();
     }
@@ -104,10 +114,9 @@ __device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds) {
 
 template <typename T, int PB>
 __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32_t *order, int64_t n,
-                                                BatchCounters *ctr, int lds_elems) {
+                                                BatchCounters *ctr, int mixed) {
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
-    (void) lds_elems;
     __shared__ int64_t s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = (int64_t) atomicAdd(&ctr->next, 1ULL);
@@ -122,7 +131,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             for (int k = 0; k < 8; ++k) ws_of(d).hdr[HDR_SUB + k] = 0;
 #endif
         const uint64_t t_start = HCAMD_TSTAMP();
-        run_temp_gemms<T>(e, lds);
+        run_temp_gemms<T>(e, lds, mixed);
         const uint64_t t_gemm = HCAMD_TSTAMP();
         ph_stack<T>(d, 0, 1);
         __syncthreads();
@@ -281,12 +290,11 @@ hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c) {
 
 template <typename T>
 hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,
-                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb) {
-    const int lds_elems = (int) (lds_bytes / sizeof(T));
+                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb, int mixed) {
     switch (pb) {
-        case 16: hipLaunchKernelGGL((k_batched<T, 16>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
-        case 8: hipLaunchKernelGGL((k_batched<T, 8>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
-        default: hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, lds_elems); break;
+        case 16: hipLaunchKernelGGL((k_batched<T, 16>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed); break;
+        case 8: hipLaunchKernelGGL((k_batched<T, 8>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed); break;
+        default: hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed); break;
     }
     return hipGetLastError();
 }
@@ -313,7 +321,7 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
     template hipError_t launch_recomp_post<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);        \
     template hipError_t launch_readjust_ddc<T>(hipStream_t, const T *, int64_t, int64_t, T *);                   \
     template hipError_t launch_batched<T>(hipStream_t, const BatchEntry<T> *, const int32_t *, int64_t,          \
-                                          BatchCounters *, int, int64_t, int);                                   \
+                                          BatchCounters *, int, int64_t, int, int);                              \
     template hipError_t set_kernel_lds_limits<T>(int64_t);
 
 HCAMD_INSTANTIATE(double)

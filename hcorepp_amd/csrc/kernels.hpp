@@ -55,7 +55,7 @@ hipError_t launch_readjust_ddc(hipStream_t s, const T *W, int64_t m, int64_t n, 
 hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c);
 template <typename T>
 hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,
-                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb);
+                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb, int mixed);
 template <typename T>
 hipError_t set_kernel_lds_limits(int64_t lds_bytes);
 

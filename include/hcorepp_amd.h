@@ -48,8 +48,9 @@ typedef enum { HCOREPP_SVD_GESVD = 0, HCOREPP_SVD_GESDD = 1 } hcorepp_svd_type;
 /* Precision form of a batch (north-star forms). */
 typedef enum {
     HCOREPP_PREC_FP64 = 0,   /* everything in FP64 */
-    HCOREPP_PREC_MIXED = 1   /* factor products + reconstruction products in FP32 (FP32 accumulate)
-                                on the terms whose rounding stays below tolerance; QR, SVD, rank in FP64 */
+    HCOREPP_PREC_MIXED = 1   /* factor products (the temporary tiles of HCore.cpp:176-264) on the FP32
+                                matrix cores with FP32 accumulation; stacking, both QRs, the core SVD, the
+                                truncation decision and the reconstruction in FP64 */
 } hcorepp_precision;
 
 /* TileMetadata + data pointer (reference include/hcorepp/operators/interface/Tile.hpp:30-52).

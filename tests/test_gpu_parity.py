@@ -223,3 +223,32 @@ def test_batched_matches_plain(ctx):
         H.HCore.gemm(1.0, As[i], Bs[i], 1.0, Cs_p[i], ctx, H.CompressionParameters(acc))
         assert rk[i] == Cs_p[i].rank
         assert rel_fro(Cs_b[i].to_dense(), Cs_p[i].to_dense()) < 1e-12
+
+
+def test_batched_mixed_precision_error_is_fp32_level(ctx):
+    """Mixed form: factor products on FP32 matrix cores with FP32 accumulation.  Its result differs
+    from the FP64 oracle by FP32 rounding of the A*B term (rel err well below 1e-5 and above FP64
+    rounding), while the FP64 batched form of the same inputs stays at FP64 rounding."""
+    H, O = _H(), _orc()
+    from hcorepp_amd.workloads import batch_ccc
+    ups = batch_ccc(16, 256, seed=7, lo=8, hi=96)
+    acc = 1e-8
+    res = {}
+    for prec in (H.PREC_FP64, H.PREC_MIXED):
+        As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
+        Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
+        Cs = [H.CompressedTile(u["cu"], u["cv"], max_rank=200) for u in ups]
+        b = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, H.CompressionParameters(acc), prec)
+        b.run()
+        b.sync_ranks_to_tiles()
+        assert b.status()[0] == 0
+        errs = []
+        for i, u in enumerate(ups):
+            oC = O.Tile.compressed(u["cu"], u["cv"], max_rank=200)
+            O.hcore_gemm(1.0, O.Tile.compressed(u["au"], u["av"]), O.Tile.compressed(u["bu"], u["bv"]), 1.0, oC,
+                         O.Params(acc))
+            errs.append(rel_fro(Cs[i].to_dense(), oC.to_dense()))
+        res[prec] = max(errs)
+        b.close()
+    assert res[H.PREC_FP64] < 1e-12
+    assert 1e-12 < res[H.PREC_MIXED] < 1e-5
