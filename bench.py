@@ -49,6 +49,104 @@ def alg_bytes(nb, ra, rb, rc, r):
     return 8.0 * nb * (2 * ra + 2 * rb + 2 * rc + 2 * r)
 
 
+def alg_flops_kind(kind, nb, ra, rb, rc, r):
+    """Algorithmic flops of one update into a compressed C of rank rc: the factor products of
+    the combination (CCC: A_V B_U and the product with B_V; DCC: A B_U; CDC: A_V B), then the
+    recompression of the stacked rank K."""
+    if kind == "CCC":
+        prod, ka = 4.0 * ra * rb * nb, ra
+    elif kind == "DCC":
+        prod, ka = 2.0 * nb * nb * rb, rb
+    else:  # CDC
+        prod, ka = 2.0 * ra * nb * nb, ra
+    K = ka + rc
+    return prod + 4.0 * (2.0 * nb * K * K - 2.0 / 3.0 * K ** 3) + 2.0 * K ** 3 + 22.0 * K ** 3 + 4.0 * nb * K * r
+
+
+def alg_bytes_kind(kind, nb, ra, rb, rc, r):
+    a = nb * nb if kind == "DCC" else 2 * nb * ra
+    b = nb * nb if kind == "CDC" else 2 * nb * rb
+    return 8.0 * (a + b + 2 * nb * rc + 2 * nb * r)
+
+
+TLR_N, TLR_NB, TLR_ELL, TLR_ACC, TLR_SEED = 16384, 1024, 0.2, 1e-10, 20261203
+
+
+def plain_tlr_leg(H, ctx, n_ctiles):
+    """Plain FP64 form, one update per launch sequence, on a 16 x 16-tile TLR covariance GEMM
+    C += A B with nb = 1024 (A = B = C = exp(-r/0.2) over 16384 points in the unit cube, diagonal
+    tiles dense, off-diagonal compressed at 1e-10, maxRank 341).  A sample of C tiles receives its
+    16 updates in k order (14 CCC, 1 DCC, 1 CDC each)."""
+    from hcorepp_amd.workloads import TLRMatrix
+    t0 = time.time()
+    M = TLRMatrix(TLR_N, TLR_NB, 3, TLR_ELL, TLR_ACC, TLR_SEED)
+    picks = [(0, 1), (9, 4), (14, 15), (3, 12)][:n_ctiles]
+    params = H.CompressionParameters(TLR_ACC)
+    dev = {}
+
+    def devtile(i, j):
+        if (i, j) not in dev:
+            t = M.tile(i, j)
+            dev[(i, j)] = H.DenseTile(t[1]) if t[0] == "D" else H.CompressedTile(t[1], t[2], max_rank=t[3])
+        return dev[(i, j)]
+
+    jobs = []
+    for (i, j) in picks:
+        tc = M.tile(i, j)
+        C = H.CompressedTile(tc[1], tc[2], max_rank=tc[3])
+        ups = []
+        for k in range(M.T):
+            A, B = devtile(i, k), devtile(k, j)
+            kind = ("D" if i == k else "C") + ("D" if k == j else "C") + "C"
+            ups.append((kind, A, B))
+        jobs.append(((i, j), C, ups))
+    log(f"[bench] plain TLR nb={TLR_NB}: {len(jobs)} C tiles, {sum(len(j[2]) for j in jobs)} updates, tiles built in "
+        f"{time.time() - t0:.1f}s")
+    # warm-up on a throw-away copy of the first C tile
+    tc = M.tile(*picks[0])
+    Cw = H.CompressedTile(tc[1], tc[2], max_rank=tc[3])
+    H.HCore.gemm(1.0, jobs[0][2][2][1], jobs[0][2][2][2], 1.0, Cw, ctx, params)
+    ctx.sync()
+    flops = byts = 0.0
+    n_up = 0
+    ranks_seen = []
+    t1 = time.perf_counter()
+    for _, C, ups in jobs:
+        for kind, A, B in ups:
+            rc = C.rank
+            ra = A.rank if kind[0] == "C" else 0
+            rb = B.rank if kind[1] == "C" else 0
+            H.HCore.gemm(1.0, A, B, 1.0, C, ctx, params)
+            flops += alg_flops_kind(kind, TLR_NB, ra, rb, rc, C.rank)
+            byts += alg_bytes_kind(kind, TLR_NB, ra, rb, rc, C.rank)
+            ranks_seen.append(C.rank)
+            n_up += 1
+    ctx.sync()
+    dt = time.perf_counter() - t1
+    # accuracy: each C tile against the dense FP64 result C + sum_k A(i,k) B(k,j)
+    errs = []
+    for (i, j), C, ups in jobs:
+        ref = M.dense_block(i, j).copy()
+        for k in range(M.T):
+            ta, tb = M.tile(i, k), M.tile(k, j)
+            a = ta[1] if ta[0] == "D" else ta[1] @ ta[2]
+            b = tb[1] if tb[0] == "D" else tb[1] @ tb[2]
+            ref += a @ b
+        errs.append(float(np.linalg.norm(C.to_dense() - ref) / np.linalg.norm(ref)))
+    tf = flops / dt / 1e12
+    out = {"updates_per_s": round(n_up / dt, 3), "updates": n_up, "seconds": round(dt, 3),
+           "fp64_tflops": round(tf, 4), "fp64_frac_of_peak": round(tf / PEAK_FP64_TFLOPS, 5),
+           "alg_hbm_gb_per_s": round(byts / dt / 1e9, 2), "ranks_min_max": [int(min(ranks_seen)), int(max(ranks_seen))],
+           "rel_err_vs_dense_fp64_max": max(errs),
+           "workload": f"{TLR_N // TLR_NB}x{TLR_N // TLR_NB}-tile TLR GEMM, nb={TLR_NB}, exp(-r/{TLR_ELL}) over "
+                       f"{TLR_N} points in the unit cube (seed {TLR_SEED}), acc {TLR_ACC}, maxRank {TLR_NB // 3}; "
+                       f"C tiles {picks}, all 16 updates each"}
+    log(f"[bench] plain TLR nb={TLR_NB}: {n_up} updates in {dt:.2f}s -> {n_up / dt:.2f} updates/s, "
+        f"{tf:.3f} TF/s FP64 ({100 * tf / PEAK_FP64_TFLOPS:.2f}% of peak), {byts / dt / 1e9:.1f} GB/s algorithmic; "
+        f"ranks {min(ranks_seen)}-{max(ranks_seen)}; rel err vs dense max {max(errs):.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +158,8 @@ def main():
     ap.add_argument("--check-sample", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--plain-tlr-tiles", type=int, default=2,
+                    help="C tiles of the nb=1024 TLR covariance GEMM timed in the plain form (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -198,6 +298,8 @@ def main():
     log(f"[bench] plain FP64 (sample {ns}): {plain_ups:.1f} updates/s; work-weighted estimate over the "
         f"batch {plain_full_est:.1f} updates/s")
 
+    plain_tlr = plain_tlr_leg(H, ctx, args.plain_tlr_tiles) if args.plain_tlr_tiles > 0 else None
+
     # ---- CPU baseline (the oracle) on a bounded sample; its outputs also check the GPU results
     from oracle import oracle as O
     ncpu = min(args.cpu_sample, args.updates)
@@ -271,6 +373,7 @@ def main():
         "rel_err_checked_updates": nchk,
         "max_jacobi_sweeps": int(sweeps),
         "mixed_form": mixed_out,
+        "plain_tlr_nb1024": plain_tlr,
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved_tflops, 4),

@@ -7,6 +7,8 @@
   i.e. U has orthonormal columns and V carries the singular values, the
   reference's compressed-tile convention (src/operators/concrete/Compressed.cpp:121-135).
 * ``batch_ccc``: independent CCC updates with ranks drawn log-uniform.
+* ``TLRMatrix``: a tile-low-rank covariance matrix built tile by tile on demand: diagonal
+  tiles dense, off-diagonal tiles compressed by the compression-constructor rule.
 
 All generation is host-side numpy with ``numpy.random.default_rng(seed)``.
 """
@@ -108,3 +110,46 @@ def batch_ccc(n: int, nb: int, seed: int, lo=8, hi=192, decay=0.85, threads: int
     finally:
         if limit is not None:
             limit.restore_original_limits()
+
+
+def compress_tile(a: np.ndarray, acc: float, max_rank: int | None = None):
+    """Dense tile -> (U, V) by the rule of the compressed-tile constructor
+    (reference src/operators/concrete/Compressed.cpp:75-146): full SVD, rank = first i >= 1 with
+    sigma_i < acc (absolute), at least 1, at most maxRank = max(min(m, n) / 3, 1); U = U_r,
+    V = Sigma_r V_r^T."""
+    m, n = a.shape
+    mr = max(min(m, n) // 3, 1) if max_rank is None else max_rank
+    u, s, vt = np.linalg.svd(a, full_matrices=False)
+    below = np.nonzero(s[1:] < acc)[0]
+    r = int(below[0]) + 1 if len(below) else len(s)
+    r = min(max(r, 1), mr)
+    return np.asfortranarray(u[:, :r]), np.asfortranarray(s[:r, None] * vt[:r]), mr
+
+
+class TLRMatrix:
+    """T x T tiles of nb rows of the covariance exp(-|x_i - x_j| / ell) over `n` points drawn
+    uniform in the unit square (dims=2) or cube (dims=3) from `seed` and sorted along a Morton
+    curve.  tile(i, j) returns ('D', dense) on the diagonal and ('C', U, V, max_rank) off it,
+    compressed at absolute accuracy `acc`; tiles are built on first use and cached."""
+
+    def __init__(self, n: int, nb: int, dims: int, ell: float, acc: float, seed: int, nugget: float = 0.0):
+        self.n, self.nb, self.acc, self.ell, self.nugget = n, nb, acc, ell, nugget
+        self.T = n // nb
+        self.points = covariance_points(n, dims, seed)
+        self._cache = {}
+
+    def dense_block(self, i: int, j: int) -> np.ndarray:
+        nb = self.nb
+        return covariance_matrix(self.points, self.ell, self.nugget, slice(i * nb, (i + 1) * nb),
+                                 slice(j * nb, (j + 1) * nb))
+
+    def tile(self, i: int, j: int):
+        key = (i, j)
+        if key not in self._cache:
+            blk = self.dense_block(i, j)
+            if i == j:
+                self._cache[key] = ("D", blk)
+            else:
+                u, v, mr = compress_tile(blk, self.acc)
+                self._cache[key] = ("C", u, v, mr)
+        return self._cache[key]
