@@ -58,7 +58,11 @@ EXPORTED = [
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
-    "hcorepp_batch_workspace_bytes", "hcorepp_batch_phase_times", "hcorepp_batch_set_grid", "hcorepp_batch_get_grid", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
+    "hcorepp_batch_workspace_bytes", "hcorepp_batch_phase_times", "hcorepp_batch_set_grid", "hcorepp_batch_get_grid",
+    "hcorepp_batch_set_rank_model", "hcorepp_batch_features", "hcorepp_batch_learned_stats",
+    "hcorepp_rank_model_features", "hcorepp_rank_model_params", "hcorepp_mlp_workspace_floats",
+    "hcorepp_mlp_forward_f32", "hcorepp_mlp_grad_f32", "hcorepp_mlp_adam_f32", "hcorepp_mlp_pack_bf16",
+    "hcorepp_mlp_predict_bf16", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
     "hcorepp_version",
 ]
 
@@ -123,6 +127,17 @@ def lib():
     L.hcorepp_batch_status.argtypes = [P, C.POINTER(I64)]
     L.hcorepp_batch_phase_times.argtypes = [P, P]
     L.hcorepp_batch_set_grid.argtypes = [P, C.c_int]
+    L.hcorepp_batch_set_rank_model.argtypes = [P, P, D, C.c_int]
+    L.hcorepp_batch_features.argtypes = [P, P]
+    L.hcorepp_batch_learned_stats.argtypes = [P, C.POINTER(I64), P]
+    L.hcorepp_rank_model_params.restype = C.c_size_t
+    L.hcorepp_mlp_workspace_floats.argtypes = [I64]
+    L.hcorepp_mlp_workspace_floats.restype = C.c_size_t
+    L.hcorepp_mlp_forward_f32.argtypes = [P, P, P, I64, P]
+    L.hcorepp_mlp_grad_f32.argtypes = [P, P, P, P, I64, P, P, C.POINTER(C.c_float)]
+    L.hcorepp_mlp_adam_f32.argtypes = [P, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_float, I64]
+    L.hcorepp_mlp_pack_bf16.argtypes = [P, P, P]
+    L.hcorepp_mlp_predict_bf16.argtypes = [P, P, P, I64, P]
     L.hcorepp_batch_get_grid.argtypes = [P]
     L.hcorepp_batch_get_grid.restype = C.c_int
     L.hcorepp_batch_workspace_bytes.argtypes = [P]
@@ -444,6 +459,25 @@ class Batch:
 
     def grid(self) -> int:
         return lib().hcorepp_batch_get_grid(self.h)
+
+    def set_rank_model(self, model_bf16, tol: float = 0.01, margin: int = 8):
+        """Learned core sizing with a BF16 parameter buffer (DeviceBuffer) or None to switch off.
+        Rewrites the device descriptors; a captured graph stays valid."""
+        ptr = model_bf16.ptr if model_bf16 is not None else None
+        self._rank_model = model_bf16
+        _check(lib().hcorepp_batch_set_rank_model(self.h, ptr, float(tol), int(margin)))
+
+    def features(self) -> np.ndarray:
+        out = np.zeros((self.n, lib().hcorepp_rank_model_features()), dtype=np.float32)
+        _check(lib().hcorepp_batch_features(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def learned_stats(self):
+        """(fallbacks of the last pass, predicted rank per update)."""
+        fb = C.c_int64(0)
+        pred = np.zeros(self.n, dtype=np.int64)
+        _check(lib().hcorepp_batch_learned_stats(self.h, C.byref(fb), pred.ctypes.data_as(C.c_void_p)))
+        return int(fb.value), pred
 
     def phase_times(self) -> np.ndarray:
         """Per-update phase timestamps and sub-phase durations of the last pass (n x 16,

@@ -13,6 +13,7 @@
 
 #include "../../include/hcorepp_amd.h"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 using namespace hcamd;
 
@@ -398,6 +399,7 @@ struct hcorepp_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> hdr_off;  // byte offset of each update's workspace header in ws
+    std::vector<char> ent_host;    // host copy of the entries (re-uploaded when the rank model changes)
 };
 
 extern "C" {
@@ -628,6 +630,10 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
             d.kmax = (int32_t) kmax;
             d.truncated_svd = p.truncated_svd;
             d.fixed_rank = (int32_t) p.fixed_rank;
+            d.rb = (b->type == HCOREPP_COMPRESSED) ? (int32_t) b->rank : 0;
+            d.rank_model = nullptr;
+            d.model_tol = 0.0;
+            d.model_margin = 0;
             d.alpha = alpha;
             d.beta = beta;
             d.accuracy = p.accuracy;
@@ -686,6 +692,7 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
         bt->hdr_off.push_back((int64_t) ((char *) e.rd.ws - bt->ws) +
                               (int64_t) ws_layout<T>(e.rd.m, e.rd.n, e.rd.kmax, e.rd.max_rank).hdr * (int64_t) sizeof(T));
     }
+    bt->ent_host.assign((const char *) ent.data(), (const char *) ent.data() + sizeof(BatchEntry<T>) * (size_t) n);
     std::vector<int32_t> order((size_t) n);
     for (int64_t i = 0; i < n; ++i) order[(size_t) i] = (int32_t) i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
@@ -801,6 +808,95 @@ hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps) {
 }
 
 size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b) { return b ? b->ws_bytes : 0; }
+
+hcorepp_status hcorepp_batch_set_rank_model(hcorepp_batch *b, const uint16_t *model_bf16, double tol, int margin) {
+    if (!b || (model_bf16 && (!(tol > 0.0) || margin < 0))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    BatchEntry<double> *ent = reinterpret_cast<BatchEntry<double> *>(b->ent_host.data());
+    for (int64_t i = 0; i < b->n; ++i) {
+        ent[i].rd.rank_model = model_bf16;
+        ent[i].rd.model_tol = tol;
+        ent[i].rd.model_margin = margin;
+    }
+    if (b->n > 0) {
+        HIP_TRY(hipMemcpyAsync(b->entries, ent, sizeof(BatchEntry<double>) * (size_t) b->n, hipMemcpyHostToDevice,
+                               b->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    }
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_features(hcorepp_batch *b, float *features) {
+    if (!b || (!features && b->n > 0)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    for (int64_t i = 0; i < b->n; ++i)
+        HIP_TRY(hipMemcpy(features + (size_t) RANK_FEATURES * i, b->ws + b->hdr_off[(size_t) i] + HDR_FEAT * 8,
+                          sizeof(float) * RANK_FEATURES, hipMemcpyDeviceToHost));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_batch_learned_stats(hcorepp_batch *b, int64_t *fallbacks, int64_t *predicted) {
+    if (!b) return fail(HCOREPP_ERR_INVALID, "null batch");
+    HIP_TRY(hipMemcpyAsync(b->ctx->pinned, b->ctr, sizeof(BatchCounters), hipMemcpyDeviceToHost, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    const BatchCounters *c = reinterpret_cast<const BatchCounters *>(b->ctx->pinned);
+    if (fallbacks) *fallbacks = (int64_t) c->fallbacks;
+    if (predicted) {
+        for (int64_t i = 0; i < b->n; ++i)
+            HIP_TRY(hipMemcpy(predicted + i, b->ws + b->hdr_off[(size_t) i] + HDR_PRED * 8, sizeof(int64_t),
+                              hipMemcpyDeviceToHost));
+    }
+    return HCOREPP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rank predictor training (FP32 on the device).
+int hcorepp_rank_model_features(void) { return RANK_FEATURES; }
+size_t hcorepp_rank_model_params(void) { return (size_t) RANK_PARAMS; }
+size_t hcorepp_mlp_workspace_floats(int64_t n) { return (size_t) mlp_workspace_floats(n); }
+
+hcorepp_status hcorepp_mlp_forward_f32(hcorepp_ctx *ctx, const float *params, const float *X, int64_t n, float *ws) {
+    if (!ctx || !params || !ws || (n > 0 && !X)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(launch_mlp_forward(ctx->stream, params, X, n, ws));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_mlp_grad_f32(hcorepp_ctx *ctx, const float *params, const float *X, const float *y, int64_t n,
+                                    float *ws, float *grads, float *loss) {
+    if (!ctx || !params || !ws || !grads || n <= 0 || !X || !y) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(launch_mlp_forward(ctx->stream, params, X, n, ws));
+    HIP_TRY(launch_mlp_backward(ctx->stream, params, X, y, n, ws, grads));
+    if (loss) {
+        const int64_t nb = mlp_loss_blocks(n);
+        std::vector<float> part((size_t) nb);
+        HIP_TRY(hipMemcpyAsync(part.data(), mlp_loss_partials(ws, n), sizeof(float) * (size_t) nb,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        double t = 0.0;
+        for (float v : part) t += v;
+        *loss = (float) t;
+    }
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_mlp_adam_f32(hcorepp_ctx *ctx, float *params, const float *grads, float *m, float *v, float lr,
+                                    float beta1, float beta2, float eps, int64_t step) {
+    if (!ctx || !params || !grads || !m || !v || step < 1) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(launch_mlp_adam(ctx->stream, params, grads, m, v, lr, beta1, beta2, eps, step));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_mlp_pack_bf16(hcorepp_ctx *ctx, const float *params, uint16_t *out) {
+    if (!ctx || !params || !out) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(launch_mlp_pack_bf16(ctx->stream, params, out));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_mlp_predict_bf16(hcorepp_ctx *ctx, const uint16_t *model_bf16, const float *X, int64_t n,
+                                        float *out) {
+    if (!ctx || !model_bf16 || (n > 0 && (!X || !out))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(launch_mlp_predict_bf16(ctx->stream, model_bf16, X, n, out));
+    return HCOREPP_OK;
+}
 
 hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b) {
     if (!b) return HCOREPP_OK;

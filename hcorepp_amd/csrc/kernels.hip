@@ -137,6 +137,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         __syncthreads();
         Ws<T> w = ws_of(d);
         const Dims z = dims_of(d, w);
+        if (z.K <= d.kmax) ph_features<T>(d, lds);
         uint64_t t_qru = 0, t_y = 0, t_qry = 0, t_svd = 0, t_ru = 0;
         if (z.K <= d.kmax) {
             wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds, nullptr, w.hdr);
@@ -147,7 +148,9 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             t_y = HCAMD_TSTAMP();
             ph_qr_y<T, PB>(d, lds);
             __syncthreads();
-            ph_core<T>(d, lds);
+            ph_core_size<T>(d, lds);
+            ph_learned_size<T>(d, lds);
+            ph_core_build<T>(d);
             t_qry = HCAMD_TSTAMP();
             ph_svd<T>(d, lds);
             __syncthreads();
@@ -175,6 +178,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             *d.crank = r;
             const int64_t st = w.hdr[HDR_STATUS];
             if (st) atomicOr(&ctr->status, (unsigned long long) st);
+            if (z.K <= d.kmax && w.hdr[HDR_LEARNED] == 2) atomicAdd(&ctr->fallbacks, 1ULL);
             atomicMax(&ctr->max_sweeps, (unsigned long long) w.hdr[HDR_SWEEPS]);
         }
         __syncthreads();
@@ -186,6 +190,7 @@ __global__ void k_reset_counters(BatchCounters *c) {
         c->next = 0;
         c->status = 0;
         c->max_sweeps = 0;
+        c->fallbacks = 0;
     }
 }
 

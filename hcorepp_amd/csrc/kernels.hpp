@@ -8,6 +8,7 @@
 #include "device_common.hpp"
 #include "gemm.hpp"
 #include "recompress.hpp"
+#include "learned.hpp"
 
 namespace hcamd {
 
@@ -37,7 +38,7 @@ struct BatchCounters {
     unsigned long long next;
     unsigned long long status;
     unsigned long long max_sweeps;
-    unsigned long long pad;
+    unsigned long long fallbacks;  // updates whose learned core sizing was rejected
 };
 
 int pick_pb(int64_t rows, int64_t lds_elems);

@@ -60,7 +60,10 @@ enum : int {
     HDR_KT = 5,      // columns of the truncated core kept for the SVD (<= size_s)
     HDR_T0 = 8,      // 8 phase timestamps (builds with HCAMD_PHASE_TIMING only)
     HDR_SUB = 16,    // 8 sub-phase cycle accumulators (builds with HCAMD_PHASE_TIMING only)
-    HDR_WORDS = 24,
+    HDR_FEAT = 24,   // RANK_FEATURES float32 rank-predictor features (12 words)
+    HDR_PRED = 36,   // predicted rank (learned core sizing), 0 when off
+    HDR_LEARNED = 37,  // 1: core sized by the prediction, 2: prediction rejected (fallback), 0: off
+    HDR_WORDS = 40,
 };
 enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2, STATUS_TOO_BIG = 4 };
 
@@ -80,10 +83,14 @@ struct RecompDesc {
     int32_t kmax;        // capacity for K used to lay out ws
     int32_t truncated_svd;
     int32_t fixed_rank;
-    int32_t pad;
+    int32_t rb;          // input rank of B (rank-predictor feature; 0 when B is dense)
     T alpha, beta;
     double accuracy;
     double *sigma_out;   // optional: sorted singular values (size_s entries)
+    const uint16_t *rank_model;  // BF16 rank-predictor parameters, or null (exact core sizing)
+    double model_tol;            // learned core sizing: accept when the dropped part <= model_tol * accuracy
+    int32_t model_margin;        // columns kept beyond the predicted rank
+    int32_t pad2;
 };
 
 // Workspace layout (element offsets of T) for capacity kmax and tile m x n.
@@ -175,6 +182,8 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
     if (wg == 0 && threadIdx.x == 0) {
         w.hdr[HDR_K] = K;
         w.hdr[HDR_STATUS] = (K > d.kmax) ? STATUS_TOO_BIG : 0;
+        w.hdr[HDR_PRED] = 0;
+        w.hdr[HDR_LEARNED] = 0;
     }
     if (K > d.kmax) return;
     const T *cu = d.cdata;
@@ -597,15 +606,16 @@ __device__ void ph_y(const RecompDesc<T> &d, int wg, int nwg, T *lds) {
 // j >= kt of R_Y whose total Frobenius norm is at most eps * ||R_Y||_F are dropped: the
 // singular values of J move by at most that much, i.e. by the rounding level of the
 // factorisation itself.  No truncation when the requested threshold is within a small factor
-// of that level, and never below a requested fixed rank.  Writes kt to the header, builds
+// of that level, and never below a requested fixed rank.
+// ph_core_size writes kt to the header and leaves in lds: rn[j] = ||R_Y[j:, :]||_F^2 (kk values)
+// followed by a lower bound of the rank threshold; ph_core_build then builds
 // J = R_Y[0:kt, :]^T (mu x kt, ld mu) and V1 = I (kt x kt).  `lds` holds >= kk + NWAVES values.
 template <typename T>
-__device__ void ph_core(const RecompDesc<T> &d, T *lds) {
+__device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int mu = z.mu, kk = z.kk, n = d.n, l = lane_id();
-    T *rn = lds;         // rn[j] = ||R_Y[j, j:mu]||^2, then suffix sums
-    T *red = lds + kk;   // NWAVES
+    T *rn = lds;  // suffix sums, then the threshold bound at rn[kk]
     for (int j = wave_id(); j < kk; j += NWAVES) {
         T sum = T(0);
         for (int c = j + l; c < mu; c += WAVE) {
@@ -641,10 +651,16 @@ __device__ void ph_core(const RecompDesc<T> &d, T *lds) {
         }
         if (kt < 1) kt = kk < 1 ? 0 : 1;
         w.hdr[HDR_KT] = kt;
-        red[0] = T(kt);
+        rn[kk] = thr_lo;
     }
     __syncthreads();
-    const int kt = (int) red[0];
+}
+
+template <typename T>
+__device__ void ph_core_build(const RecompDesc<T> &d) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const int mu = z.mu, kt = z.kt, n = d.n;
     for (int64_t e = threadIdx.x; e < (int64_t) mu * kt; e += NT) {
         const int64_t i = e % mu, j = e / mu;
         w.j[e] = (i >= j) ? w.y[j + i * n] : T(0);
@@ -654,6 +670,12 @@ __device__ void ph_core(const RecompDesc<T> &d, T *lds) {
         w.v1[e] = (i == j) ? T(1) : T(0);
     }
     __syncthreads();
+}
+
+template <typename T>
+__device__ void ph_core(const RecompDesc<T> &d, T *lds) {
+    ph_core_size<T>(d, lds);
+    ph_core_build<T>(d);
 }
 
 // ---------------------------------------------------------------------------

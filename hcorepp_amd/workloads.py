@@ -153,3 +153,33 @@ class TLRMatrix:
                 u, v, mr = compress_tile(blk, self.acc)
                 self._cache[key] = ("C", u, v, mr)
         return self._cache[key]
+
+
+def batch_ccc_pool(n: int, nb: int, seed: int, lo=8, hi=192, decay=0.85, pool: int = 96):
+    """Same distribution as batch_ccc (ranks log-uniform, U orthonormal, V = diag(decay^i) Q^T)
+    but fast: every orthonormal factor is a random column subset of one of `pool` Haar-random
+    nb x nb orthogonal matrices (a column subset of a Haar matrix is a uniformly random
+    orthonormal basis); the six factors of an update come from six different pool matrices.
+    Used to generate rank-predictor training data in bulk."""
+    rng = np.random.default_rng(seed)
+    Q = []
+    for _ in range(pool):
+        q, r = np.linalg.qr(rng.standard_normal((nb, nb)))
+        Q.append(q * np.sign(np.diag(r)))
+    ra = loguniform_ranks(rng, n, lo, hi)
+    rb = loguniform_ranks(rng, n, lo, hi)
+    rc = loguniform_ranks(rng, n, lo, hi)
+    out = []
+    for i in range(n):
+        pk = rng.choice(pool, 6, replace=False)
+
+        def fac(k0, k1, r):
+            u = Q[pk[k0]][:, rng.choice(nb, r, replace=False)]
+            v = (decay ** np.arange(r))[:, None] * Q[pk[k1]][:, rng.choice(nb, r, replace=False)].T
+            return np.asfortranarray(u), np.asfortranarray(v)
+
+        au, av = fac(0, 1, int(ra[i]))
+        bu, bv = fac(2, 3, int(rb[i]))
+        cu, cv = fac(4, 5, int(rc[i]))
+        out.append(dict(au=au, av=av, bu=bu, bv=bv, cu=cu, cv=cv))
+    return out

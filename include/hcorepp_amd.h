@@ -178,6 +178,35 @@ int hcorepp_batch_get_grid(hcorepp_batch *b);
  * updates, core SVD sweeps, Q application, 3 spare).  Returns
  * HCOREPP_ERR_UNSUPPORTED unless the library was built with -DHCAMD_PHASE_TIMING. */
 hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times);
+/* ---------------------------------------------------------------------------
+ * Learned rank predictor (no counterpart in the reference).  Per-update features are computed on
+ * the device right after stacking (hcorepp_batch_features copies them, RANK_FEATURES floats per
+ * update, after a pass); an MLP of hcorepp_rank_model_params() parameters predicts log2 of the
+ * post-truncation rank.  With a model set (BF16 parameters in device memory, from
+ * hcorepp_mlp_pack_bf16), each update's core is presized to the prediction + margin columns and
+ * accepted only if the dropped rows of the pivoted R factor have Frobenius norm <= tol * threshold;
+ * otherwise the exact core is used (counted as a fallback).  model_bf16 = NULL switches it off. */
+hcorepp_status hcorepp_batch_set_rank_model(hcorepp_batch *b, const uint16_t *model_bf16, double tol, int margin);
+hcorepp_status hcorepp_batch_features(hcorepp_batch *b, float *features);
+/* Fallbacks of the last pass (synchronises); predicted (may be NULL, host, n entries) gets each
+ * update's predicted rank (0 when no model was set). */
+hcorepp_status hcorepp_batch_learned_stats(hcorepp_batch *b, int64_t *fallbacks, int64_t *predicted);
+int hcorepp_rank_model_features(void);
+size_t hcorepp_rank_model_params(void);
+/* FP32 training kernels on device buffers: X (n x features, row-major), y (n), workspace of
+ * hcorepp_mlp_workspace_floats(n) floats.  grad computes the mean-squared-error loss (to host,
+ * synchronises when loss is non-NULL) and its gradient; adam applies one Adam step (step >= 1);
+ * pack_bf16 rounds the parameters to BF16 for inference; predict_bf16 runs the in-kernel BF16
+ * inference on rows of X. */
+size_t hcorepp_mlp_workspace_floats(int64_t n);
+hcorepp_status hcorepp_mlp_forward_f32(hcorepp_ctx *ctx, const float *params, const float *X, int64_t n, float *ws);
+hcorepp_status hcorepp_mlp_grad_f32(hcorepp_ctx *ctx, const float *params, const float *X, const float *y, int64_t n,
+                                    float *ws, float *grads, float *loss);
+hcorepp_status hcorepp_mlp_adam_f32(hcorepp_ctx *ctx, float *params, const float *grads, float *m, float *v, float lr,
+                                    float beta1, float beta2, float eps, int64_t step);
+hcorepp_status hcorepp_mlp_pack_bf16(hcorepp_ctx *ctx, const float *params, uint16_t *out);
+hcorepp_status hcorepp_mlp_predict_bf16(hcorepp_ctx *ctx, const uint16_t *model_bf16, const float *X, int64_t n,
+                                        float *out);
 /* Bytes of device workspace the batch holds. */
 size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b);
 hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b);

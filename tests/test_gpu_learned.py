@@ -1,0 +1,112 @@
+"""GPU tests of the learned rank predictor: the device training kernels against the CPU oracle
+(oracle/mlp.py), the in-kernel BF16 inference, and the learned core sizing of the batched
+form (results equal the exact form to rounding; a rejected prediction falls back)."""
+import numpy as np
+import pytest
+
+from helpers import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    import hcorepp_amd as H
+    from hcorepp_amd import learned as Lm
+    from oracle import mlp as M
+    return H, Lm, M
+
+
+def test_device_gradient_matches_oracle(ctx):
+    H, Lm, M = _mods()
+    rng = np.random.default_rng(4)
+    p = (rng.standard_normal(Lm.n_params()) * 0.2).astype(np.float32)
+    X = rng.standard_normal((700, Lm.n_features())).astype(np.float32)
+    y = rng.standard_normal(700).astype(np.float32)
+    model = Lm.RankPredictor(ctx, params=p)
+    loss, g = model.grad(X, y)
+    loss_o, g_o = M.loss_grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
+    assert abs(loss - loss_o) <= 1e-4 * abs(loss_o)
+    assert np.linalg.norm(g - g_o) <= 1e-4 * np.linalg.norm(g_o)
+    assert np.max(np.abs(g - g_o)) <= 1e-4 * np.max(np.abs(g_o))
+
+
+def test_device_adam_step_matches_oracle(ctx):
+    H, Lm, M = _mods()
+    rng = np.random.default_rng(5)
+    p = (rng.standard_normal(Lm.n_params()) * 0.2).astype(np.float32)
+    X = rng.standard_normal((256, Lm.n_features())).astype(np.float32)
+    y = rng.standard_normal(256).astype(np.float32)
+    model = Lm.RankPredictor(ctx, params=p)
+    model.train(X, y, epochs=1, batch=256, lr=1e-3, seed=0)  # exactly one Adam step on all samples
+    _, g_o = M.loss_grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
+    p1, _, _ = M.adam(p.astype(np.float64), g_o, 0.0 * g_o, 0.0 * g_o, 1, 1e-3)
+    got = model.host_params()
+    assert np.max(np.abs(got - p1)) <= 2e-5
+
+
+def test_bf16_inference_matches_oracle(ctx):
+    H, Lm, M = _mods()
+    rng = np.random.default_rng(6)
+    p = (rng.standard_normal(Lm.n_params()) * 0.2).astype(np.float32)
+    X = rng.standard_normal((64, Lm.n_features())).astype(np.float32)
+    model = Lm.RankPredictor(ctx, params=p)
+    got = model.predict_log2(X)
+    want = M.predict_bf16(p, X)
+    fp32, _ = M.forward(p.astype(np.float64), X.astype(np.float64))
+    # identical BF16 rounding of weights and activations; FP32 accumulation order differs
+    assert np.max(np.abs(got - want)) <= 1e-3 * (1 + np.max(np.abs(want)))
+    # and BF16 inference stays close to the FP32 network
+    assert np.max(np.abs(got - fp32)) <= 0.05 * (1 + np.max(np.abs(fp32)))
+
+
+def _batch(H, ctx, ups, acc=1e-8, mr=200):
+    As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
+    Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
+    Cs = [H.CompressedTile(u["cu"], u["cv"], max_rank=mr) for u in ups]
+    return H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, H.CompressionParameters(acc), H.PREC_FP64), Cs
+
+
+def test_learned_core_sizing_matches_exact_and_falls_back(ctx):
+    H, Lm, M = _mods()
+    from hcorepp_amd.workloads import batch_ccc_pool
+    train = batch_ccc_pool(384, 256, seed=31, lo=8, hi=120, pool=24)
+    test = batch_ccc_pool(48, 256, seed=32, lo=8, hi=120, pool=24)
+    # training data from the exact method
+    b, _ = _batch(H, ctx, train)
+    b.run()
+    X, r = b.features(), b.ranks()
+    b.close()
+    assert np.all(np.isfinite(X))
+    model = Lm.RankPredictor(ctx, seed=2)
+    hist = model.train(X, np.log2(r), epochs=60, batch=128, lr=3e-3)
+    assert hist[-1] < 0.5 * hist[0]
+    # exact run of the test set
+    be, Ce = _batch(H, ctx, test)
+    be.run()
+    re = be.sync_ranks_to_tiles()
+    exact = [c.to_dense() for c in Ce]
+    # learned core sizing with the trained model
+    bl, Cl = _batch(H, ctx, test)
+    bl.set_rank_model(model.bf16(), tol=0.01, margin=8)
+    bl.run()
+    rl = bl.sync_ranks_to_tiles()
+    fb, pred = bl.learned_stats()
+    assert np.all(pred > 0)
+    for i in range(len(test)):
+        assert rl[i] == re[i]
+        assert rel_fro(Cl[i].to_dense(), exact[i]) < 1e-9
+    # a model that predicts rank 1 everywhere: every presized core that drops anything is rejected
+    zeros = np.zeros(Lm.n_params(), dtype=np.float32)
+    bad = Lm.RankPredictor(ctx, params=zeros)
+    bb, Cb = _batch(H, ctx, test)
+    bb.set_rank_model(bad.bf16(), tol=0.01, margin=0)
+    bb.run()
+    rb = bb.sync_ranks_to_tiles()
+    fb_bad, pred_bad = bb.learned_stats()
+    assert np.all(pred_bad == 1)
+    assert fb_bad > 0
+    for i in range(len(test)):
+        assert rb[i] == re[i]
+        assert rel_fro(Cb[i].to_dense(), exact[i]) < 1e-12
+    for x in (be, bl, bb):
+        x.close()

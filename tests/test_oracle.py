@@ -246,3 +246,49 @@ def test_compression_constructor():
     r = next(i for i in range(1, len(s)) if s[i] < 1e-6)
     assert t.rank == r
     assert rel_fro(t.to_dense(), a) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Rank-predictor network oracle (oracle/mlp.py), pinned by finite differences and closed forms.
+def _mlp_case(seed=3, n=40):
+    from oracle import mlp as M
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(M.F * M.H + M.H + M.H * M.H + M.H + M.H + 1) * 0.3
+    X = rng.standard_normal((n, M.F))
+    y = rng.standard_normal(n)
+    return M, p, X, y
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    M, p, X, y = _mlp_case()
+    loss, g = M.loss_grad(p, X, y)
+    rng = np.random.default_rng(9)
+    idx = rng.choice(p.size, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        num = (M.loss_grad(p + e, X, y)[0] - M.loss_grad(p - e, X, y)[0]) / (2 * h)
+        assert abs(num - g[i]) <= 1e-6 * max(1.0, abs(num)), (i, num, g[i])
+
+
+def test_mlp_oracle_closed_forms():
+    """All-zero hidden weights: output = b3 for every sample, so the loss is mean((b3 - y)^2) and
+    only b3 has a gradient, 2 mean(b3 - y)."""
+    M, p, X, y = _mlp_case()
+    q = np.zeros_like(p)
+    q[-1] = 0.7
+    loss, g = M.loss_grad(q, X, y)
+    assert np.isclose(loss, np.mean((0.7 - y) ** 2))
+    assert np.allclose(g[:-1], 0.0) and np.isclose(g[-1], 2 * np.mean(0.7 - y))
+    # one Adam step from zero moments moves every parameter with a nonzero gradient by ~lr
+    p1, m, v = M.adam(q, g, np.zeros_like(q), np.zeros_like(q), 1, 1e-3)
+    assert np.isclose(abs(p1[-1] - q[-1]), 1e-3, rtol=1e-4)
+
+
+def test_bf16_rounding_oracle():
+    from oracle import mlp as M
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e38], dtype=np.float32)
+    r = M.bf16(x)
+    # 1 + 2^-8 is a tie: rounds to even (1.0); 1 + 3*2^-8 rounds up to 1 + 2^-6
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 2 ** -6) and r[3] == -2.5
