@@ -460,9 +460,10 @@ class Batch:
     def grid(self) -> int:
         return lib().hcorepp_batch_get_grid(self.h)
 
-    def set_rank_model(self, model_bf16, tol: float = 0.01, margin: int = 8):
-        """Learned core sizing with a BF16 parameter buffer (DeviceBuffer) or None to switch off.
-        Rewrites the device descriptors; a captured graph stays valid."""
+    def set_rank_model(self, model_bf16, tol: float = 0.1, margin: int = 40):
+        """Learned core sizing with a BF16 parameter buffer (DeviceBuffer).  model_bf16=None with
+        tol > 0 applies the tolerance directly to the pivoted R factor; None with tol=0 switches
+        both off.  Rewrites the device descriptors; a captured graph stays valid."""
         ptr = model_bf16.ptr if model_bf16 is not None else None
         self._rank_model = model_bf16
         _check(lib().hcorepp_batch_set_rank_model(self.h, ptr, float(tol), int(margin)))

@@ -810,7 +810,7 @@ hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps) {
 size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b) { return b ? b->ws_bytes : 0; }
 
 hcorepp_status hcorepp_batch_set_rank_model(hcorepp_batch *b, const uint16_t *model_bf16, double tol, int margin) {
-    if (!b || (model_bf16 && (!(tol > 0.0) || margin < 0))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    if (!b || tol < 0.0 || (model_bf16 && (!(tol > 0.0) || margin < 0))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
     BatchEntry<double> *ent = reinterpret_cast<BatchEntry<double> *>(b->ent_host.data());
     for (int64_t i = 0; i < b->n; ++i) {
         ent[i].rd.rank_model = model_bf16;

@@ -136,12 +136,33 @@ __device__ inline float rank_mlp_wave(const uint16_t *P, const float *x) {
 // core is below that bound and the rank rule cannot pick a rank above kc); otherwise the
 // exact width kt stays (fallback).  Header: HDR_PRED = prediction, HDR_LEARNED = 1 accepted,
 // 2 fallback.  `lds` is the ph_core_size scratch (suffix sums, threshold bound).
+//
+// Without a model but with model_tol > 0, the same tolerance is applied directly to the pivoted
+// R factor (the smallest kc with ||R_Y[kc:, :]||_F <= model_tol * threshold; HDR_LEARNED = 3): the
+// rule the prediction is measured against.
 template <typename T>
 __device__ void ph_learned_size(const RecompDesc<T> &d, T *lds) {
-    if (!d.rank_model || d.fixed_rank > 0) return;
+    if (d.fixed_rank > 0 || !(d.model_tol > 0.0)) return;
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int kk = z.kk;
+    if (!d.rank_model) {
+        if (threadIdx.x == 0) {
+            const T *rn = lds;
+            const T lim = T(d.model_tol) * rn[kk];
+            int kc = z.kt;
+            for (int j = 1; j < z.kt; ++j) {
+                if (sqrt(rn[j]) <= lim) {
+                    kc = j;
+                    break;
+                }
+            }
+            w.hdr[HDR_KT] = kc;
+            w.hdr[HDR_LEARNED] = 3;
+        }
+        __syncthreads();
+        return;
+    }
     __shared__ float s_pred;
     if (wave_id() == 0) {
         const float o = rank_mlp_wave(d.rank_model, reinterpret_cast<const float *>(w.hdr + HDR_FEAT));

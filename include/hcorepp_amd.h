@@ -185,7 +185,9 @@ hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times);
  * post-truncation rank.  With a model set (BF16 parameters in device memory, from
  * hcorepp_mlp_pack_bf16), each update's core is presized to the prediction + margin columns and
  * accepted only if the dropped rows of the pivoted R factor have Frobenius norm <= tol * threshold;
- * otherwise the exact core is used (counted as a fallback).  model_bf16 = NULL switches it off. */
+ * otherwise the exact core is used (counted as a fallback).  model_bf16 = NULL with tol > 0 applies
+ * the same tolerance directly to the pivoted R factor (no prediction); NULL with tol = 0 switches
+ * both off (exact eps-level core). */
 hcorepp_status hcorepp_batch_set_rank_model(hcorepp_batch *b, const uint16_t *model_bf16, double tol, int margin);
 hcorepp_status hcorepp_batch_features(hcorepp_batch *b, float *features);
 /* Fallbacks of the last pass (synchronises); predicted (may be NULL, host, n entries) gets each
