@@ -240,4 +240,184 @@ __device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int
     return sweep;
 }
 
+// ---------------------------------------------------------------------------
+// Workgroup-cooperative variant: the four waves work on the same tile of TW columns, each
+// owning an interleaved quarter of the rows of J (row lane + 64 (wave + 4 t), t < RPL) and of V
+// (t < RPLV).  A round reduce-scatters each wave's partial dot products, exchanges them
+// through LDS (double-buffered by round parity, one barrier per round) and every wave derives
+// the same rotations from the same sums taken in the same order.  A wave only ever reads the
+// rows it wrote, so consecutive tiles need no barrier.  Wider tiles than one wave's registers
+// allow mean fewer passes over J and V per sweep (traffic ~ ncols / (TW / 2) passes).
+// ---------------------------------------------------------------------------
+template <typename T, int RPL, int RPLV, int TW, bool FULL>
+__device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], int rd, T tol, T floor2, T *red,
+                                          int parity) {
+    constexpr int NP = TW / 2;
+    constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
+    const int l = lane_id(), w = wave_id();
+    T part[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const T a = y[t][p], b = y[t][q];
+            part[3 * k] += a * a;
+            part[3 * k + 1] += b * b;
+            part[3 * k + 2] += a * b;
+        }
+    }
+    const T tot = wave_multi_sum_fast<T, NV>(part);
+    T *buf = red + parity * (NWAVES * NV);
+    const int idx = wave_multi_sum_index<NV>(l);
+    if (l == multi_sum_src_lane<NV>(idx)) buf[w * NV + idx] = tot;
+    __syncthreads();
+    T c = T(1), s = T(0);
+    if (l < NP) {
+        T app = T(0), aqq = T(0), apq = T(0);
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVES; ++w2) {
+            app += buf[w2 * NV + 3 * l];
+            aqq += buf[w2 * NV + 3 * l + 1];
+            apq += buf[w2 * NV + 3 * l + 2];
+        }
+        jacobi_rotation(app, aqq, apq, tol, floor2, c, s);
+    }
+    int any = 0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const T sk = read_lane(s, k);
+        if (sk != T(0)) {
+            const T ck = read_lane(c, k);
+            const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const T a = y[r][p], b = y[r][q];
+                y[r][p] = ck * a - sk * b;
+                y[r][q] = sk * a + ck * b;
+            }
+#pragma unroll
+            for (int r = 0; r < RPLV; ++r) {
+                const T va = vt[r][p], vb = vt[r][q];
+                vt[r][p] = ck * va - sk * vb;
+                vt[r][q] = sk * va + ck * vb;
+            }
+            any = 1;
+        }
+    }
+    return any;
+}
+
+template <typename T, int TW>
+__host__ __device__ constexpr int coop_jacobi_lds_elems() {
+    return 2 * NWAVES * ((3 * (TW / 2) <= 4) ? 4 : (3 * (TW / 2) <= 8) ? 8 : (3 * (TW / 2) <= 16) ? 16 : 32) +
+           NWAVES + 2;
+}
+
+template <typename T, int RPL, int RPLV, int TW>
+__device__ int coop_jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ, bool full,
+                                T tol, T floor2, T *red, int &parity) {
+    constexpr int B = TW / 2;
+    const int l = lane_id(), w = wave_id();
+    T y[RPL][TW], vt[RPLV][TW];
+#pragma unroll
+    for (int c = 0; c < TW; ++c) {
+        const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int r = l + WAVE * (w + NWAVES * t);
+            y[t][c] = (r < rows && gc < ncols) ? J[r + (int64_t) gc * ldj] : T(0);
+        }
+#pragma unroll
+        for (int t = 0; t < RPLV; ++t) {
+            const int r = l + WAVE * (w + NWAVES * t);
+            vt[t][c] = (r < ncols && gc < ncols) ? V[r + (int64_t) gc * ldv] : T(0);
+        }
+    }
+    int any = 0;
+    if (full) {
+#pragma unroll
+        for (int rd = 0; rd < TW - 1; ++rd) {
+            any |= coop_round<T, RPL, RPLV, TW, true>(y, vt, rd, tol, floor2, red, parity);
+            parity ^= 1;
+        }
+    } else {
+#pragma unroll
+        for (int rd = 0; rd < TW / 2; ++rd) {
+            any |= coop_round<T, RPL, RPLV, TW, false>(y, vt, rd, tol, floor2, red, parity);
+            parity ^= 1;
+        }
+    }
+    if (any) {
+#pragma unroll
+        for (int c = 0; c < TW; ++c) {
+            const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = l + WAVE * (w + NWAVES * t);
+                if (r < rows && gc < ncols) J[r + (int64_t) gc * ldj] = y[t][c];
+            }
+#pragma unroll
+            for (int t = 0; t < RPLV; ++t) {
+                const int r = l + WAVE * (w + NWAVES * t);
+                if (r < ncols && gc < ncols) V[r + (int64_t) gc * ldv] = vt[t][c];
+            }
+        }
+    }
+    return any;
+}
+
+// Whole-workgroup block Jacobi with cooperative tiles: rows <= 256 RPL, ncols <= 256 RPLV.
+// Same contract as wg_block_jacobi; `gbuf` holds coop_jacobi_lds_elems<T, TW>() elements.
+template <typename T, int RPL, int RPLV, int TW>
+__device__ int wg_coop_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int max_sweeps, T *gbuf) {
+    constexpr int B = TW / 2;
+    constexpr int NV = (3 * B <= 4) ? 4 : (3 * B <= 8) ? 8 : (3 * B <= 16) ? 16 : 32;
+    T *red = gbuf;                                // 2 x NWAVES x NV
+    T *s_red = gbuf + 2 * NWAVES * NV;            // NWAVES
+    T part = T(0);
+    for (int c = 0; c < ncols; ++c)
+        for (int r = threadIdx.x; r < rows; r += NT) {
+            const T a = J[r + (int64_t) c * ldj];
+            part += a * a;
+        }
+    const T fro2 = wg_sum(part, s_red);
+    const T floor2 = Eps<T>::value * Eps<T>::value * fro2;
+    const T tol = sqrt(T(rows)) * Eps<T>::value;
+    const int nblk = (ncols + B - 1) / B;
+    const int nbp = nblk + (nblk & 1);
+    int parity = 0;
+    int sweep = 0;
+    if (ncols <= 1) return 0;
+    for (; sweep < max_sweeps; ++sweep) {
+        int any = 0;
+        for (int step = 0; step < nbp - 1; ++step) {
+            for (int bp = 0; bp < nbp / 2; ++bp) {
+                int bI, bJ;
+                if (bp == 0) {
+                    bI = nbp - 1;
+                    bJ = step;
+                } else {
+                    bI = (step + bp) % (nbp - 1);
+                    bJ = (step - bp + nbp - 1) % (nbp - 1);
+                }
+                if (bI > bJ) {
+                    const int t = bI;
+                    bI = bJ;
+                    bJ = t;
+                }
+                if (bI * B >= ncols) continue;
+                any |= coop_jacobi_tile<T, RPL, RPLV, TW>(J, rows, ncols, ldj, V, ldv, bI * B, bJ * B, step == 0,
+                                                          tol, floor2, red, parity);
+            }
+        }
+        // every wave derived the same rotations, so `any` agrees across the workgroup
+        if (!any) break;
+    }
+    __syncthreads();
+    return sweep;
+}
+
 }  // namespace hcamd

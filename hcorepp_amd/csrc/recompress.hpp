@@ -686,8 +686,16 @@ __device__ HCAMD_PHASE_ATTR int block_jacobi_nl(T *J, int rows, int ncols, T *V,
     return wg_block_jacobi<T, RPL, RPLV, TW>(J, rows, ncols, rows, V, ncols, JACOBI_MAX_SWEEPS, gbuf);
 }
 
-// Tile shape by core rows and (truncated) width: J rows per lane, V rows per lane and tile
-// width, chosen so that both register tiles fit 256 VGPRs without spilling.
+template <typename T, int RPL, int RPLV, int TW>
+__device__ HCAMD_PHASE_ATTR int coop_jacobi_nl(T *J, int rows, int ncols, T *V, T *gbuf) {
+    return wg_coop_jacobi<T, RPL, RPLV, TW>(J, rows, ncols, rows, V, ncols, JACOBI_MAX_SWEEPS, gbuf);
+}
+
+// Tile shape by core rows and (truncated) width.  Per-wave tiles (a wave owns a whole tile of J
+// and V columns; four tiles in flight per workgroup) with shapes that fit 256 VGPRs without
+// spilling; for 193-256 rows the four waves share 16-column tiles instead (rows split across
+// waves, one barrier per rotation round), which measured faster there because it needs half the
+// passes over J and V of the widest per-wave tile that fits.
 template <typename T>
 __device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
     if (rows <= 64) return block_jacobi_nl<T, 1, 1, 16>(J, rows, ncols, V, gbuf);
@@ -697,9 +705,7 @@ __device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
     if (rows <= 192)
         return ncols <= 128 ? block_jacobi_nl<T, 3, 2, 8>(J, rows, ncols, V, gbuf)
                             : block_jacobi_nl<T, 3, 3, 4>(J, rows, ncols, V, gbuf);
-    if (rows <= 256)
-        return ncols <= 192 ? block_jacobi_nl<T, 4, 3, 4>(J, rows, ncols, V, gbuf)
-                            : block_jacobi_nl<T, 4, 4, 4>(J, rows, ncols, V, gbuf);
+    if (rows <= 256) return coop_jacobi_nl<T, 1, 1, 16>(J, rows, ncols, V, gbuf);
     if (rows <= 320)
         return ncols <= 192 ? block_jacobi_nl<T, 5, 3, 4>(J, rows, ncols, V, gbuf)
                             : block_jacobi_nl<T, 5, 5, 4>(J, rows, ncols, V, gbuf);
@@ -721,9 +727,10 @@ __device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
 constexpr int JACOBI_ROWS_MAX = 768;
 // LDS of the core-SVD phase: ph_core's row norms (<= JACOBI_ROWS_MAX + NWAVES) and the block Jacobi's
 // reduction slots, which reuse the same space.
-constexpr int JACOBI_LDS_ELEMS =
-        (JACOBI_ROWS_MAX + NWAVES + 8 > block_jacobi_lds_elems<double, 16>() + 8) ? JACOBI_ROWS_MAX + NWAVES + 8
-                                                                                   : block_jacobi_lds_elems<double, 16>() + 8;
+constexpr int JACOBI_LDS_ELEMS = JACOBI_ROWS_MAX + NWAVES + 8;
+static_assert(JACOBI_LDS_ELEMS >= block_jacobi_lds_elems<double, 16>() &&
+                      JACOBI_LDS_ELEMS >= coop_jacobi_lds_elems<double, 16>(),
+              "core-SVD LDS covers both Jacobi variants");
 static_assert(JACOBI_LDS_ELEMS >= JACOBI_ROWS_MAX + NWAVES, "ph_core scratch must fit the core-SVD LDS");
 
 template <typename T>
