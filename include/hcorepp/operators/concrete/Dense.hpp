@@ -18,8 +18,8 @@ public:
               const kernels::RunContext &aContext, bool aMemoryOwnership = true) {
         (void) aLayout;
         (void) aContext;
-        const int64_t k = (int64_t) std::min(aNumOfRows, aNumOfCols);
-        this->mDesc = hcorepp_tile{(int64_t) aNumOfRows, (int64_t) aNumOfCols, k, k, (int64_t) aLeadingDim, 0,
+        // a dense tile carries no rank (its metadata reports 0, as the reference's does)
+        this->mDesc = hcorepp_tile{(int64_t) aNumOfRows, (int64_t) aNumOfCols, 0, 0, (int64_t) aLeadingDim, 0,
                                    HCOREPP_DENSE, nullptr};
         if (!aMemoryOwnership) {
             this->mDesc.data = aPdata;
@@ -33,6 +33,11 @@ public:
     DenseTile(size_t aNumOfRows, size_t aNumOfCols, T *aPdata, size_t aLeadingDim,
               const kernels::RunContext &aContext)
         : DenseTile(aNumOfRows, aNumOfCols, aPdata, aLeadingDim, blas::Layout::ColMajor, aContext) {}
+    // Non-owning tile over caller memory described by aMeta (PackTile, Dense.cpp:136-142).
+    DenseTile(const TileMetadata &aMeta, T *apDeviceData) {
+        this->mDesc = hcorepp_tile{(int64_t) aMeta.mNumOfRows, (int64_t) aMeta.mNumOfCols, 0, 0,
+                                   (int64_t) aMeta.mLeadingDimension, 0, HCOREPP_DENSE, apDeviceData};
+    }
     T *GetTileSubMatrix(size_t aIndex) const {
         if (aIndex != 0) throw std::invalid_argument("DenseTile::GetTileSubMatrix: index must be 0");
         return this->GetData();

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstring>
+#include <utility>
 #include <stdexcept>
 
 #include <hcorepp/common/Definitions.hpp>
@@ -43,6 +44,14 @@ public:
     hcorepp_tile &Desc() { return mDesc; }
     const hcorepp_tile &Desc() const { return mDesc; }
     T *GetData() const { return static_cast<T *>(mDesc.data); }
+    // Metadata + data pointer of the tile (reference Tile::UnPackTile, Dense.cpp:127-134,
+    // Compressed.cpp:769-779): the caller owns the returned TileMetadata; the data stays the tile's.
+    std::pair<TileMetadata *, T *> UnPackTile(const kernels::RunContext &) const {
+        return {new TileMetadata((size_t) mDesc.rows, (size_t) mDesc.cols, (size_t) mDesc.rank,
+                                 (size_t) mDesc.max_rank, (size_t) mDesc.ld, blas::Layout::ColMajor,
+                                 mDesc.type == HCOREPP_DENSE ? DENSE : COMPRESSED),
+                GetData()};
+    }
 
 protected:
     void Allocate(size_t aElements) {

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include <hcorepp/api/HCore.hpp>
+#include <hcorepp/operators/interface/TilePacker.hpp>
 
 using namespace hcorepp::api;
 using namespace hcorepp::operators;
@@ -45,6 +46,17 @@ int main(int argc, char **argv) {
             }
             err = std::max(err, std::abs(out[i + m * j] - s));
         }
-    std::printf("dropin_compile: max abs err %.3e flops %zu\n", err, flops);
-    return err < 1e-10 ? 0 : 1;
+    // TilePacker round trip (TestTilePacker.cpp): metadata as in the reference, zero-copy repack
+    auto md = TilePacker<double>::UnPackTile(A, ctx);
+    bool ok = md.first->mNumOfRows == m && md.first->mNumOfCols == n && md.first->mMatrixRank == r &&
+              md.first->mType == COMPRESSED && md.second == A.GetUMatrix();
+    auto mdd = TilePacker<double>::UnPackTile(C, ctx);
+    ok = ok && mdd.first->mMatrixRank == 0 && mdd.first->mType == DENSE && mdd.first->mLeadingDimension == m;
+    Tile<double> *packed = TilePacker<double>::PackTile(*md.first, md.second, ctx);
+    ok = ok && packed->isCompressed() && packed->GetData() == A.GetUMatrix();
+    delete packed;
+    delete md.first;
+    delete mdd.first;
+    std::printf("dropin_compile: max abs err %.3e flops %zu packer %s\n", err, flops, ok ? "ok" : "FAIL");
+    return (err < 1e-10 && ok) ? 0 : 1;
 }
