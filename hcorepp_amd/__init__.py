@@ -62,7 +62,8 @@ EXPORTED = [
     "hcorepp_batch_set_rank_model", "hcorepp_batch_features", "hcorepp_batch_learned_stats",
     "hcorepp_rank_model_features", "hcorepp_rank_model_params", "hcorepp_mlp_workspace_floats",
     "hcorepp_mlp_forward_f32", "hcorepp_mlp_grad_f32", "hcorepp_mlp_adam_f32", "hcorepp_mlp_pack_bf16",
-    "hcorepp_mlp_predict_bf16", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
+    "hcorepp_mlp_predict_bf16", "hcorepp_potrf_d", "hcorepp_trsm_batch_d", "hcorepp_syrk_batch_d",
+    "hcorepp_transpose_batch_d", "hcorepp_batch_destroy", "hcorepp_debug_workspace", "hcorepp_last_error",
     "hcorepp_version",
 ]
 
@@ -138,6 +139,10 @@ def lib():
     L.hcorepp_mlp_adam_f32.argtypes = [P, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_float, I64]
     L.hcorepp_mlp_pack_bf16.argtypes = [P, P, P]
     L.hcorepp_mlp_predict_bf16.argtypes = [P, P, P, I64, P]
+    L.hcorepp_potrf_d.argtypes = [P, C.POINTER(_Tile)]
+    L.hcorepp_trsm_batch_d.argtypes = [P, I64, C.POINTER(_Tile), C.POINTER(_Tile)]
+    L.hcorepp_syrk_batch_d.argtypes = [P, I64, C.POINTER(_Tile), C.POINTER(_Tile)]
+    L.hcorepp_transpose_batch_d.argtypes = [P, I64, C.POINTER(_Tile), C.POINTER(_Tile)]
     L.hcorepp_batch_get_grid.argtypes = [P]
     L.hcorepp_batch_get_grid.restype = C.c_int
     L.hcorepp_batch_workspace_bytes.argtypes = [P]

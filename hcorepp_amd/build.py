@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libhcorepp_amd.so")
-SOURCES = ["kernels.hip", "mlp.hip", "api.hip"]
+SOURCES = ["kernels.hip", "mlp.hip", "cholesky.hip", "api.hip"]
 ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -14,6 +14,7 @@
 #include "../../include/hcorepp_amd.h"
 #include "kernels.hpp"
 #include "mlp.hpp"
+#include "cholesky.hpp"
 
 using namespace hcamd;
 
@@ -845,6 +846,107 @@ hcorepp_status hcorepp_batch_learned_stats(hcorepp_batch *b, int64_t *fallbacks,
             HIP_TRY(hipMemcpy(predicted + i, b->ws + b->hdr_off[(size_t) i] + HDR_PRED * 8, sizeof(int64_t),
                               hipMemcpyDeviceToHost));
     }
+    return HCOREPP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// TLR Cholesky tile operations (FP64).
+extern "C++" {
+namespace {
+template <typename J>
+hcorepp_status upload_jobs(hcorepp_ctx *ctx, const std::vector<J> &jobs, J **dev) {
+    *dev = (J *) arena_take(ctx, sizeof(J) * std::max<size_t>(jobs.size(), 1));
+    if (!jobs.empty()) HIP_TRY(hipMemcpy(*dev, jobs.data(), sizeof(J) * jobs.size(), hipMemcpyHostToDevice));
+    return HCOREPP_OK;
+}
+}  // namespace
+}
+
+hcorepp_status hcorepp_potrf_d(hcorepp_ctx *ctx, hcorepp_tile *A) {
+    hcorepp_status st;
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    if ((st = validate_tile<double>(A, "A"))) return st;
+    if (A->type != HCOREPP_DENSE || A->rows != A->cols)
+        return fail(HCOREPP_ERR_INVALID, "potrf needs a square dense tile");
+    if ((st = arena_begin(ctx, 4096))) return st;
+    int *info = (int *) arena_take(ctx, sizeof(int));
+    HIP_TRY(launch_potrf(ctx->stream, (double *) A->data, (int) A->rows, A->ld, info));
+    int h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (h) return fail(HCOREPP_ERR_NUMERIC, "potrf: tile not positive definite (pivot %d)", h);
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_trsm_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *L, hcorepp_tile *B) {
+    hcorepp_status st;
+    if (!ctx || n < 0 || (n > 0 && !B)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    if ((st = validate_tile<double>(L, "L"))) return st;
+    if (L->type != HCOREPP_DENSE || L->rows != L->cols) return fail(HCOREPP_ERR_INVALID, "L must be square dense");
+    std::vector<CholTrsm> jobs;
+    for (int64_t i = 0; i < n; ++i) {
+        if ((st = validate_tile<double>(&B[i], "B[i]"))) return st;
+        if (B[i].type != HCOREPP_COMPRESSED || B[i].cols != L->rows)
+            return fail(HCOREPP_ERR_INVALID, "trsm: B[%lld] must be compressed with cols == L rows", (long long) i);
+        if (B[i].rank == 0) continue;
+        jobs.push_back(CholTrsm{(double *) B[i].data + B[i].rows * B[i].max_rank, (const double *) L->data,
+                                B[i].rank, L->rows, L->ld});
+    }
+    if ((st = arena_begin(ctx, sizeof(CholTrsm) * (jobs.size() + 1) + 4096))) return st;
+    CholTrsm *dj;
+    if ((st = upload_jobs(ctx, jobs, &dj))) return st;
+    HIP_TRY(launch_trsm_v(ctx->stream, dj, (int) jobs.size()));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_syrk_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *A, hcorepp_tile *C) {
+    hcorepp_status st;
+    if (!ctx || n < 0 || (n > 0 && (!A || !C))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    std::vector<CholSyrk> jobs;
+    size_t scratch = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if ((st = validate_tile<double>(&A[i], "A[i]")) || (st = validate_tile<double>(&C[i], "C[i]"))) return st;
+        if (A[i].type != HCOREPP_COMPRESSED || C[i].type != HCOREPP_DENSE || C[i].rows != C[i].cols ||
+            A[i].rows != C[i].rows)
+            return fail(HCOREPP_ERR_INVALID, "syrk: A[%lld] compressed, C[%lld] square dense of A's rows",
+                        (long long) i, (long long) i);
+        scratch += aligned(sizeof(double) * (size_t) (A[i].rank * A[i].rank)) +
+                   aligned(sizeof(double) * (size_t) (A[i].rows * A[i].rank));
+    }
+    if ((st = arena_begin(ctx, scratch + sizeof(CholSyrk) * (size_t) (n + 1) + 4096))) return st;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t r = A[i].rank, m = A[i].rows;
+        double *w = (double *) arena_take(ctx, sizeof(double) * (size_t) (r * r));
+        double *t = (double *) arena_take(ctx, sizeof(double) * (size_t) (m * r));
+        const double *u = (const double *) A[i].data;
+        jobs.push_back(CholSyrk{(double *) C[i].data, u, u + m * A[i].max_rank, w, t, m, r, C[i].ld});
+    }
+    CholSyrk *dj;
+    if ((st = upload_jobs(ctx, jobs, &dj))) return st;
+    HIP_TRY(launch_syrk(ctx->stream, dj, (int) jobs.size()));
+    return HCOREPP_OK;
+}
+
+hcorepp_status hcorepp_transpose_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *src, hcorepp_tile *dst) {
+    hcorepp_status st;
+    if (!ctx || n < 0 || (n > 0 && (!src || !dst))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    std::vector<CholTrans> jobs;
+    for (int64_t i = 0; i < n; ++i) {
+        if ((st = validate_tile<double>(&src[i], "src[i]")) || (st = validate_tile<double>(&dst[i], "dst[i]")))
+            return st;
+        if (src[i].type != HCOREPP_COMPRESSED || dst[i].type != HCOREPP_COMPRESSED || dst[i].rows != src[i].cols ||
+            dst[i].cols != src[i].rows || dst[i].max_rank < src[i].rank)
+            return fail(HCOREPP_ERR_INVALID, "transpose: dst[%lld] must be a compressed %lldx%lld tile of capacity "
+                        ">= %lld", (long long) i, (long long) src[i].cols, (long long) src[i].rows,
+                        (long long) src[i].rank);
+        dst[i].rank = src[i].rank;
+        jobs.push_back(CholTrans{(const double *) src[i].data, (double *) dst[i].data, src[i].rows, src[i].cols,
+                                 src[i].rank, src[i].max_rank, dst[i].max_rank});
+    }
+    if ((st = arena_begin(ctx, sizeof(CholTrans) * (jobs.size() + 1) + 4096))) return st;
+    CholTrans *dj;
+    if ((st = upload_jobs(ctx, jobs, &dj))) return st;
+    HIP_TRY(launch_transpose_factors(ctx->stream, dj, (int) jobs.size()));
     return HCOREPP_OK;
 }
 

@@ -6,11 +6,6 @@
 
 namespace hcamd {
 
-// Every engine kernel runs 256 threads and is register-capped for two waves per SIMD
-// (256 VGPRs per lane), so that two workgroups of the batched kernel share a CU: the
-// latency-bound Jacobi rounds and panel reductions of one overlap the other's.  The
-// device functions the kernels share inherit the cap only when every caller has it.
-#define HCAMD_KERNEL __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
 
 // Phase timestamps of the batched kernel (100 MHz constant clock), recorded only in builds
 // with HCAMD_PHASE_TIMING; read back with hcorepp_batch_phase_times.

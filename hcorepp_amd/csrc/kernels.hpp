@@ -12,6 +12,12 @@
 
 namespace hcamd {
 
+// Every engine kernel runs 256 threads and is register-capped for two waves per SIMD
+// (256 VGPRs per lane), so that two workgroups of the batched kernel share a CU: the
+// latency-bound Jacobi rounds and panel reductions of one overlap the other's.  The
+// device functions the kernels share inherit the cap only when every caller has it.
+#define HCAMD_KERNEL __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+
 // One temporary factor product W = op(A) * op(B) (alpha 1, beta 0), as the
 // reference's temporary DenseTile GEMMs (src/api/HCore.cpp:176-264).
 template <typename T>

@@ -143,6 +143,10 @@ class TLRMatrix:
         return covariance_matrix(self.points, self.ell, self.nugget, slice(i * nb, (i + 1) * nb),
                                  slice(j * nb, (j + 1) * nb))
 
+    def dense_block_range(self, r0: int, r1: int) -> np.ndarray:
+        """Dense rows/cols [r0, r1) of the matrix (with the nugget on the diagonal)."""
+        return covariance_matrix(self.points, self.ell, self.nugget, slice(r0, r1), slice(r0, r1))
+
     def tile(self, i: int, j: int):
         key = (i, j)
         if key not in self._cache:
@@ -183,3 +187,20 @@ def batch_ccc_pool(n: int, nb: int, seed: int, lo=8, hi=192, decay=0.85, pool: i
         cu, cv = fac(4, 5, int(rc[i]))
         out.append(dict(au=au, av=av, bu=bu, bv=bv, cu=cu, cv=cv))
     return out
+
+
+def prefetch_tiles(M: "TLRMatrix", keys, threads: int = 8):
+    """Build (and cache) the listed tiles of a TLRMatrix in parallel (one BLAS thread each)."""
+    import concurrent.futures as cf
+    keys = [k for k in keys if k not in M._cache]
+    try:
+        from threadpoolctl import threadpool_limits
+        limit = threadpool_limits(limits=1)
+    except Exception:  # pragma: no cover
+        limit = None
+    try:
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            list(ex.map(lambda k: M.tile(*k), keys))
+    finally:
+        if limit is not None:
+            limit.restore_original_limits()

@@ -179,6 +179,24 @@ int hcorepp_batch_get_grid(hcorepp_batch *b);
  * HCOREPP_ERR_UNSUPPORTED unless the library was built with -DHCAMD_PHASE_TIMING. */
 hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times);
 /* ---------------------------------------------------------------------------
+ * TLR Cholesky tile operations (FP64, lower, A = L L^T), batched over independent tiles where
+ * the factorisation allows it.  The trailing updates A_ij -= L_ik L_jk^T are hcorepp_gemm_d /
+ * batch updates with alpha = -1 whose B operand is the transposed tile from
+ * hcorepp_transpose_batch_d.
+ * Replace HCore<T>::Potrf (reference src/api/HCore.cpp:586-622, dense tile, Uplo::Lower): */
+hcorepp_status hcorepp_potrf_d(hcorepp_ctx *ctx, hcorepp_tile *A);
+/* Replace HCore<T>::Trsm (HCore.cpp:624-647; Side::Right, Uplo::Lower, Op::Trans, NonUnit, alpha 1) on
+ * n compressed tiles B[i] = U V: V <- V L^{-T}. */
+hcorepp_status hcorepp_trsm_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *L, hcorepp_tile *B);
+/* Replace HCore<T>::Syrk (HCore.cpp:484-584; compressed A, alpha -1, beta 1) on n dense tiles:
+ * C[i] <- C[i] - A[i] A[i]^T (full tile written). */
+hcorepp_status hcorepp_syrk_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *A, hcorepp_tile *C);
+/* Compressed dst[i] = src[i]^T (U_dst = V_src^T, V_dst = U_src^T; dst[i].rank is set).  No
+ * counterpart in the reference, whose aCholesky Gemm branch (HCore.cpp:160-174) re-plumbs the
+ * operands instead. */
+hcorepp_status hcorepp_transpose_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *src, hcorepp_tile *dst);
+
+/* ---------------------------------------------------------------------------
  * Learned rank predictor (no counterpart in the reference).  Per-update features are computed on
  * the device right after stacking (hcorepp_batch_features copies them, RANK_FEATURES floats per
  * update, after a pass); an MLP of hcorepp_rank_model_params() parameters predicts log2 of the

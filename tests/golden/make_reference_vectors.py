@@ -6,7 +6,8 @@ Sources (ecrc/hcorepp):
   tests/operators/TestCompressedTile.cpp:124-260   "Compressed Tile Gemm"
   tests/kernels/TestKernels.cpp               ProcessVpointer :157-192, CalculateNewRank :194-237,
                                               CalculateUVptr :239-265, CalculateVTnew :267-329,
-                                              LaCpy :383-468, geqrf :470-511, Laset :513-604
+                                              LaCpy :383-468, geqrf :470-511, Laset :513-604,
+                                              Potrf :1070-1124, Trsm :1126-1183, CrossProduct (syrk) :1185-1248
 The reference validates products with a 1 % relative tolerance
 (tests/helpers/src/testHelpers.cpp:83-86); exact integer kernels are compared exactly.
 """
@@ -76,6 +77,18 @@ KERNELS = dict(
     laset_lower=dict(src="tests/kernels/TestKernels.cpp:578-604", m=4, n=4, off=1.0, diag=2.0,
                      expected=[2, 1, 1, 1, 0, 2, 1, 1, 0, 0, 2, 1, 0, 0, 0, 2]),
     svd_3x3=dict(src="tests/kernels/TestKernels.cpp:696-790", a=[[1, 2, 3], [4, 5, 6], [7, 8, 9]]),
+    # Cholesky tile kernels (row-major literals as written there)
+    potrf_lower=dict(src="tests/kernels/TestKernels.cpp:1070-1124", a=[[25, 15, 5], [15, 13, 11], [5, 11, 21]],
+                     expected=[[5, 0, 0], [3, 2, 0], [1, 4, 2]]),
+    trsm_left_upper=dict(src="tests/kernels/TestKernels.cpp:1126-1183", a=[[1, 2, 3], [0, 1, 1], [0, 0, 2]],
+                         b=[[8], [4], [2]], expected=[[-1], [3], [1]]),
+    syrk_lower=dict(src="tests/kernels/TestKernels.cpp:1185-1248",
+                    a=[[3.12393, -1.16854, -0.304408, -2.15901], [-1.16854, 1.86968, 1.04094, 1.35925],
+                       [-0.304408, 1.04094, 4.43374, 1.21072], [-2.15901, 1.35925, 1.21072, 5.57265]],
+                    expected=[[15.878412787064, -1.16854, -0.304408, -2.15901],
+                              [-9.08673783542, 7.7923056801, 1.04094, 1.35925],
+                              [-6.13095182416, 8.56286609912, 22.300113620064, 1.21072],
+                              [-20.73289403456, 13.8991634747, 14.18705411188, 39.0291556835]]),
 )
 
 if __name__ == "__main__":

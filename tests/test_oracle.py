@@ -292,3 +292,79 @@ def test_bf16_rounding_oracle():
     r = M.bf16(x)
     # 1 + 2^-8 is a tie: rounds to even (1.0); 1 + 3*2^-8 rounds up to 1 + 2^-6
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 2 ** -6) and r[3] == -2.5
+
+
+# ---------------------------------------------------------------- Cholesky tile operations
+def _chol_oracle():
+    from oracle import cholesky as OC
+    return OC
+
+
+def test_oracle_potrf_known_answer():
+    v = K["potrf_lower"]
+    L = _chol_oracle().potrf(np.array(v["a"], dtype=float))
+    assert np.array_equal(L, np.array(v["expected"], dtype=float))
+
+
+def test_oracle_trsm_known_answer():
+    """The reference's Trsm vector solves A x = b with A upper (Side Left); the tile operation
+    solves X L^T = V with L lower.  With J the index reversal, L = J A J is lower and
+    x^T = (J b)^T L^{-T} reversed."""
+    v = K["trsm_left_upper"]
+    A = np.array(v["a"], dtype=float)
+    b = np.array(v["b"], dtype=float)[:, 0]
+    L = A[::-1, ::-1].copy()
+    x = _chol_oracle().trsm_v(L, b[::-1][None, :])[0][::-1]
+    assert np.allclose(x, np.array(v["expected"], dtype=float)[:, 0], rtol=0, atol=1e-14)
+
+
+def test_oracle_syrk_known_answer():
+    """CrossProduct vector: lower(A A^T).  The compressed syrk with U = A, V = I, C = 0 gives
+    lower(-A A^T)."""
+    v = K["syrk_lower"]
+    A = np.array(v["a"], dtype=float)
+    out = _chol_oracle().syrk_compressed(np.zeros((4, 4)), A, np.eye(4))
+    exp = np.tril(np.array(v["expected"], dtype=float))
+    assert np.allclose(-out, exp, rtol=1e-11, atol=0)
+    assert np.array_equal(np.triu(out, 1), np.zeros((4, 4)))
+
+
+def _small_tlr(acc=1e-5, n=512, nb=128):
+    from hcorepp_amd.workloads import TLRMatrix
+    M = TLRMatrix(n, nb, 2, 0.1, acc, 20261205, nugget=1e-2)
+    diag = [M.tile(k, k)[1] for k in range(M.T)]
+    off = {(i, k): M.tile(i, k)[1:] for i in range(M.T) for k in range(i)}
+    return M, diag, off
+
+
+def test_oracle_tlr_cholesky_backward_error():
+    """A TLR Cholesky at accuracy 1e-5 of an SPD covariance reproduces A to O(acc): the
+    relative backward error is below 0.2 acc and L is close to numpy's dense Cholesky."""
+    OC = _chol_oracle()
+    M, diag, off = _small_tlr()
+    F = OC.TLRCholesky(diag, off, M.acc).factor()
+    Ld = F.dense_factor()
+    A = M.dense_block_range(0, M.n)
+    be = np.linalg.norm(A - Ld @ Ld.T) / np.linalg.norm(A)
+    assert be < 2e-6, be
+    Lx = np.linalg.cholesky(A)
+    assert rel_fro(Ld, Lx) < 1e-4
+    assert all(0 < r <= 42 for r in F.ranks().values())
+
+
+def test_oracle_tlr_cholesky_planted_sign_slip_fails():
+    """The backward-error check catches a trailing update applied with the wrong sign."""
+    OC = _chol_oracle()
+    M, diag, off = _small_tlr()
+    F = OC.TLRCholesky(diag, off, M.acc)
+    orig = OC.syrk_compressed
+    try:
+        OC.syrk_compressed = lambda c, u, v: np.tril(c + (u @ (v @ v.T)) @ u.T)
+        with pytest.raises(np.linalg.LinAlgError):
+            F.factor()
+            Ld = F.dense_factor()
+            A = M.dense_block_range(0, M.n)
+            if np.linalg.norm(A - Ld @ Ld.T) / np.linalg.norm(A) > 1e-3:
+                raise np.linalg.LinAlgError("backward error")
+    finally:
+        OC.syrk_compressed = orig
