@@ -251,7 +251,8 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
             const int64_t kmax = arank + C->rank;
             const WsLayout L = ws_layout<T>(C->rows, C->cols, kmax, C->max_rank);
             need += aligned(sizeof(T) * L.total) + aligned(sizeof(int64_t) * 8) +
-                    aligned(sizeof(double) * (size_t) std::max<int64_t>(kmax, 1));
+                    aligned(sizeof(double) * (size_t) std::max<int64_t>(kmax, 1)) +
+                    aligned(sizeof(int32_t) * JACOBI_MAX_SWEEPS);
         }
     }
     if ((st = arena_begin(ctx, need + 4096))) return st;
@@ -355,7 +356,8 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
                                   L.y, L.t_u, L.t_y, L.j, L.v1, L.vtmp};
         for (int i = 0; i < 16; ++i) ctx->last_info[i] = info[i];
     }
-    HIP_TRY(launch_recomp_pre<T>(s, d, K, kLdsBytes, pb));
+    int32_t *jflags = (int32_t *) arena_take(ctx, sizeof(int32_t) * JACOBI_MAX_SWEEPS);
+    HIP_TRY(launch_recomp_pre<T>(s, d, K, kLdsBytes, pb, jflags, reinterpret_cast<int32_t *>(ctx->pinned + HDR_WORDS)));
     // The plain form reads the new rank back with one stream synchronisation,
     // as the reference does (src/kernels/cuda/CudaKernels.cu:656-697).
     const int64_t *hdr = reinterpret_cast<const int64_t *>(d.ws + L.hdr);

@@ -369,6 +369,162 @@ __device__ int coop_jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, in
     return any;
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative tile with deferred V: the rotations of the J tile are recorded in LDS and applied
+// to the V columns once the tile's rounds are done, streaming V rows through registers (one row
+// of the tile per thread at a time).  Only the J tile stays in registers, so 16-column tiles fit
+// for up to 768 rows (RPL <= 3) without spilling.  Used by the multi-workgroup Jacobi, where
+// each workgroup owns one tile per launch.
+// ---------------------------------------------------------------------------
+template <typename T, int TW>
+__host__ __device__ constexpr int coop_rec_lds_elems() {
+    return coop_jacobi_lds_elems<T, TW>() + 2 * (TW - 1) * (TW / 2);
+}
+
+template <typename T, int RPL, int TW, bool FULL>
+__device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T floor2, T *red, int parity,
+                                              T *cs) {
+    constexpr int NP = TW / 2;
+    constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
+    const int l = lane_id(), w = wave_id();
+    T part[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const T a = y[t][p], b = y[t][q];
+            part[3 * k] += a * a;
+            part[3 * k + 1] += b * b;
+            part[3 * k + 2] += a * b;
+        }
+    }
+    const T tot = wave_multi_sum_fast<T, NV>(part);
+    T *buf = red + parity * (NWAVES * NV);
+    const int idx = wave_multi_sum_index<NV>(l);
+    if (l == multi_sum_src_lane<NV>(idx)) buf[w * NV + idx] = tot;
+    __syncthreads();
+    T c = T(1), s = T(0);
+    if (l < NP) {
+        T app = T(0), aqq = T(0), apq = T(0);
+#pragma unroll
+        for (int w2 = 0; w2 < NWAVES; ++w2) {
+            app += buf[w2 * NV + 3 * l];
+            aqq += buf[w2 * NV + 3 * l + 1];
+            apq += buf[w2 * NV + 3 * l + 2];
+        }
+        jacobi_rotation(app, aqq, apq, tol, floor2, c, s);
+        if (w == 0) {
+            cs[2 * (rd * NP + l)] = c;
+            cs[2 * (rd * NP + l) + 1] = s;
+        }
+    }
+    int any = 0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const T sk = read_lane(s, k);
+        if (sk != T(0)) {
+            const T ck = read_lane(c, k);
+            const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const T a = y[r][p], b = y[r][q];
+                y[r][p] = ck * a - sk * b;
+                y[r][q] = sk * a + ck * b;
+            }
+            any = 1;
+        }
+    }
+    return any;
+}
+
+// Apply the recorded rotations of one tile (rounds of the FULL or cross ordering) to the V
+// columns [cI, cI+TW/2) and [cJ, cJ+TW/2) (V: ncols x ncols, ld ldv), one V row per thread.
+template <typename T, int TW, bool FULL>
+__device__ __forceinline__ void coop_apply_v(T *V, int ncols, int64_t ldv, int cI, int cJ, const T *cs) {
+    constexpr int B = TW / 2, NP = TW / 2, ROUNDS = FULL ? TW - 1 : TW / 2;
+#pragma unroll 1
+    for (int r = threadIdx.x; r < ncols; r += NT) {
+        T v[TW];
+#pragma unroll
+        for (int c = 0; c < TW; ++c) {
+            const int gc = (c < B) ? cI + c : cJ + (c - B);
+            v[c] = gc < ncols ? V[r + (int64_t) gc * ldv] : T(0);
+        }
+#pragma unroll
+        for (int rd = 0; rd < ROUNDS; ++rd) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const T sk = cs[2 * (rd * NP + k) + 1];
+                const T ck = cs[2 * (rd * NP + k)];
+                const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+                const T a = v[p], b = v[q];
+                v[p] = ck * a - sk * b;
+                v[q] = sk * a + ck * b;
+            }
+            // keep the next round's rotation loads from being scheduled early (all would be live)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int c = 0; c < TW; ++c) {
+            const int gc = (c < B) ? cI + c : cJ + (c - B);
+            if (gc < ncols) V[r + (int64_t) gc * ldv] = v[c];
+        }
+    }
+}
+
+// One tile of the multi-workgroup Jacobi; `lds` holds coop_rec_lds_elems<T, TW>() elements.
+// Returns whether any pair was rotated (uniform over the workgroup).
+template <typename T, int RPL, int TW>
+__device__ int coop_jacobi_tile_rec(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ,
+                                    bool full, T tol, T floor2, T *lds) {
+    constexpr int B = TW / 2;
+    T *red = lds;
+    T *cs = lds + coop_jacobi_lds_elems<T, TW>();
+    const int l = lane_id(), w = wave_id();
+    T y[RPL][TW];
+#pragma unroll
+    for (int c = 0; c < TW; ++c) {
+        const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int r = l + WAVE * (w + NWAVES * t);
+            y[t][c] = (r < rows && gc < ncols) ? J[r + (int64_t) gc * ldj] : T(0);
+        }
+    }
+    int any = 0, parity = 0;
+    if (full) {
+#pragma unroll
+        for (int rd = 0; rd < TW - 1; ++rd) {
+            any |= coop_round_rec<T, RPL, TW, true>(y, rd, tol, floor2, red, parity, cs);
+            parity ^= 1;
+        }
+    } else {
+#pragma unroll
+        for (int rd = 0; rd < TW / 2; ++rd) {
+            any |= coop_round_rec<T, RPL, TW, false>(y, rd, tol, floor2, red, parity, cs);
+            parity ^= 1;
+        }
+    }
+    if (any) {
+#pragma unroll
+        for (int c = 0; c < TW; ++c) {
+            const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = l + WAVE * (w + NWAVES * t);
+                if (r < rows && gc < ncols) J[r + (int64_t) gc * ldj] = y[t][c];
+            }
+        }
+        __syncthreads();  // the last round's rotations in cs are visible to every wave
+        if (full) coop_apply_v<T, TW, true>(V, ncols, ldv, cI, cJ, cs);
+        else coop_apply_v<T, TW, false>(V, ncols, ldv, cI, cJ, cs);
+    }
+    return any;
+}
+
 // Whole-workgroup block Jacobi with cooperative tiles: rows <= 256 RPL, ncols <= 256 RPLV.
 // Same contract as wg_block_jacobi; `gbuf` holds coop_jacobi_lds_elems<T, TW>() elements.
 template <typename T, int RPL, int RPLV, int TW>

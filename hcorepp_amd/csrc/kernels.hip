@@ -50,6 +50,116 @@ __global__ void HCAMD_KERNEL k_svd(RecompDesc<T> d) {
     ph_svd<T>(d, gbuf);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-workgroup core SVD of the plain (one update per launch sequence) form.  A large core
+// (hundreds of columns) is the whole cost of an update, so its one-sided block Jacobi is spread
+// over the GPU: each launch is one step of the round-robin ordering of column blocks, one
+// workgroup per block pair (the four waves of a workgroup share the pair's 16 columns, rows
+// split across the waves).  flags[s] records whether sweep s rotated anything; the launches of
+// a later sweep return at once when the previous sweep was already rotation-free.
+// ---------------------------------------------------------------------------
+constexpr int MW_TW = 16;
+
+template <typename T>
+__global__ void HCAMD_KERNEL k_core(RecompDesc<T> d, int32_t *flags) {
+    __shared__ T gbuf[JACOBI_LDS_ELEMS];
+    ph_core<T>(d, gbuf);
+    for (int i = threadIdx.x; i < JACOBI_MAX_SWEEPS; i += NT) flags[i] = 0;
+    if (threadIdx.x == 0) {
+        Ws<T> w = ws_of(d);
+        const Dims z = dims_of(d, w);
+        // gbuf[j] = ||R_Y[j:, :]||_F^2 after ph_core_size, so ||J||_F^2 = gbuf[0] - gbuf[kt]
+        const T fro2 = z.kk > 0 ? gbuf[0] - (z.kt < z.kk ? gbuf[z.kt] : T(0)) : T(0);
+        w.hdr[HDR_JFRO2] = __double_as_longlong((double) fro2);
+    }
+}
+
+template <typename T, int RPL>
+__global__ void HCAMD_KERNEL k_jacobi_step(RecompDesc<T> d, int32_t *flags, int sweep, int step) {
+    constexpr int B = MW_TW / 2;
+    __shared__ T lds[coop_rec_lds_elems<T, MW_TW>()];
+    if (sweep > 0 && flags[sweep - 1] == 0) return;  // converged in an earlier sweep
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const int rows = z.mu, ncols = z.kt;
+    if (ncols <= 1) return;
+    const int nblk = (ncols + B - 1) / B;
+    const int nbp = nblk + (nblk & 1);
+    const int bp = (int) blockIdx.x;
+    if (step >= nbp - 1 || bp >= nbp / 2) return;
+    int bI, bJ;
+    if (bp == 0) {
+        bI = nbp - 1;
+        bJ = step;
+    } else {
+        bI = (step + bp) % (nbp - 1);
+        bJ = (step - bp + nbp - 1) % (nbp - 1);
+    }
+    if (bI > bJ) {
+        const int t = bI;
+        bI = bJ;
+        bJ = t;
+    }
+    if (bI * B >= ncols) return;
+    const T fro2 = (T) __longlong_as_double(w.hdr[HDR_JFRO2]);
+    const T floor2 = Eps<T>::value * Eps<T>::value * fro2;
+    const T tol = sqrt(T(rows)) * Eps<T>::value;
+    const int any = coop_jacobi_tile_rec<T, RPL, MW_TW>(w.j, rows, ncols, rows, w.v1, ncols, bI * B, bJ * B,
+                                                         step == 0, tol, floor2, lds);
+    if (any && threadIdx.x == 0) flags[sweep] = 1;
+}
+
+template <typename T>
+__global__ void HCAMD_KERNEL k_svd_finish(RecompDesc<T> d, const int32_t *flags, int launched) {
+    int sweeps = launched;
+    for (int s = 0; s < launched; ++s)
+        if (flags[s] == 0) {
+            sweeps = s;
+            break;
+        }
+    ph_svd_finish<T>(d, sweeps);
+}
+
+template <typename T, int RPL>
+static void launch_jacobi_sweep(hipStream_t s, const RecompDesc<T> &d, int32_t *flags, int sweep, int steps,
+                                int grid) {
+    for (int st = 0; st < steps; ++st)
+        hipLaunchKernelGGL((k_jacobi_step<T, RPL>), dim3(grid), dim3(NT), 0, s, d, flags, sweep, st);
+}
+
+// Core SVD over many workgroups: core build, Jacobi sweeps launched in chunks (the host reads the
+// last sweep's flag between chunks, through the pinned word `host_flag`), then the rank rule.
+template <typename T>
+static hipError_t launch_core_svd_mw(hipStream_t s, const RecompDesc<T> &d, int mu, int kk, int32_t *flags,
+                                     volatile int32_t *host_flag) {
+    hipLaunchKernelGGL(k_core<T>, dim3(1), dim3(NT), 0, s, d, flags);
+    constexpr int B = MW_TW / 2;
+    const int nblk = (kk + B - 1) / B;
+    const int nbp = nblk + (nblk & 1);
+    const int steps = nbp - 1, grid = nbp / 2;
+    const int rpl = (mu + NT - 1) / NT;
+    int launched = 0, chunk = 6;
+    while (launched < JACOBI_MAX_SWEEPS) {
+        const int end = launched + chunk < JACOBI_MAX_SWEEPS ? launched + chunk : JACOBI_MAX_SWEEPS;
+        for (int sw = launched; sw < end; ++sw) {
+            if (rpl <= 1) launch_jacobi_sweep<T, 1>(s, d, flags, sw, steps, grid);
+            else if (rpl == 2) launch_jacobi_sweep<T, 2>(s, d, flags, sw, steps, grid);
+            else launch_jacobi_sweep<T, 3>(s, d, flags, sw, steps, grid);
+        }
+        launched = end;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = hipMemcpyAsync((void *) host_flag, flags + launched - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        if (*host_flag == 0) break;
+        chunk = 2;
+    }
+    hipLaunchKernelGGL(k_svd_finish<T>, dim3(1), dim3(NT), 0, s, d, flags, launched);
+    return hipGetLastError();
+}
+
 template <typename T, int PB>
 __global__ void HCAMD_KERNEL k_recon(RecompDesc<T> d, int cols_per_wg) {
     extern __shared__ unsigned char smem_raw[];
@@ -243,9 +353,11 @@ static int clamp_grid(int64_t work) {
 }
 
 template <typename T>
-hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb) {
+hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb,
+                             int32_t *jflags, int32_t *host_flag) {
     const int64_t m = d.m, n = d.n;
     const int64_t mu = m < K ? m : K;
+    const int64_t kk = n < mu ? n : mu;
     hipLaunchKernelGGL(k_stack<T>, dim3(clamp_grid(m * K)), dim3(NT), 0, s, d);
     hipError_t e = launch_qr<T>(s, d, lds_bytes, pb, 0);
     if (e != hipSuccess) return e;
@@ -253,6 +365,8 @@ hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, i
     hipLaunchKernelGGL(k_y<T>, dim3(ytiles), dim3(NT), 0, s, d);
     e = launch_qr<T>(s, d, lds_bytes, pb, 1);
     if (e != hipSuccess) return e;
+    if (jflags && host_flag && kk >= MW_MIN_COLS && mu <= JACOBI_ROWS_MAX)
+        return launch_core_svd_mw<T>(s, d, (int) mu, (int) kk, jflags, host_flag);
     hipLaunchKernelGGL(k_svd<T>, dim3(1), dim3(NT), 0, s, d);
     return hipGetLastError();
 }
@@ -317,7 +431,8 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
 #define HCAMD_INSTANTIATE(T)                                                                                     \
     template hipError_t launch_gemm<T>(hipStream_t, int, int, int, T, const T *, int64_t, bool, const T *,      \
                                        int64_t, bool, T, T *, int64_t);                                          \
-    template hipError_t launch_recomp_pre<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);         \
+    template hipError_t launch_recomp_pre<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int, int32_t *, \
+                                             int32_t *);                                                         \
     template hipError_t launch_recomp_post<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);        \
     template hipError_t launch_readjust_ddc<T>(hipStream_t, const T *, int64_t, int64_t, T *);                   \
     template hipError_t launch_batched<T>(hipStream_t, const BatchEntry<T> *, const int32_t *, int64_t,          \

@@ -58,6 +58,7 @@ enum : int {
     HDR_SWEEPS = 3,  // Jacobi sweeps used
     HDR_STATUS = 4,  // status bits (STATUS_*)
     HDR_KT = 5,      // columns of the truncated core kept for the SVD (<= size_s)
+    HDR_JFRO2 = 6,   // ||J||_F^2 of the truncated core as double bits (multi-workgroup Jacobi)
     HDR_T0 = 8,      // 8 phase timestamps (builds with HCAMD_PHASE_TIMING only)
     HDR_SUB = 16,    // 8 sub-phase cycle accumulators (builds with HCAMD_PHASE_TIMING only)
     HDR_FEAT = 24,   // RANK_FEATURES float32 rank-predictor features (12 words)
@@ -733,20 +734,14 @@ static_assert(JACOBI_LDS_ELEMS >= block_jacobi_lds_elems<double, 16>() &&
               "core-SVD LDS covers both Jacobi variants");
 static_assert(JACOBI_LDS_ELEMS >= JACOBI_ROWS_MAX + NWAVES, "ph_core scratch must fit the core-SVD LDS");
 
+// Tail of the core SVD once the Jacobi has converged (`sweeps` used): singular values, their
+// descending order, the rank rule and the header words.
 template <typename T>
-__device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
+__device__ void ph_svd_finish(const RecompDesc<T> &d, int sweeps) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int mu = z.mu, kk = z.kk, kt = z.kt, K = z.K;
     __shared__ int s_nan;
-    int sweeps = 0;
-    if (mu > JACOBI_ROWS_MAX) {
-        if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
-    } else {
-        HCAMD_SUB_BEGIN(t_jac);
-        sweeps = run_block_jacobi<T>(w.j, mu, kt, w.v1, lds);
-        HCAMD_SUB_END(t_jac, w.hdr, 3);
-    }
     if (threadIdx.x == 0) s_nan = 0;
     __syncthreads();
     const int l = lane_id();
@@ -807,6 +802,21 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
         }
     }
     __syncthreads();
+}
+
+template <typename T>
+__device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    int sweeps = 0;
+    if (z.mu > JACOBI_ROWS_MAX) {
+        if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
+    } else {
+        HCAMD_SUB_BEGIN(t_jac);
+        sweeps = run_block_jacobi<T>(w.j, z.mu, z.kt, w.v1, lds);
+        HCAMD_SUB_END(t_jac, w.hdr, 3);
+    }
+    ph_svd_finish<T>(d, sweeps);
 }
 
 // ---------------------------------------------------------------------------

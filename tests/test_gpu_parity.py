@@ -126,6 +126,9 @@ CASES = [
     ("CCC", 200, 30, 25, 40, 100, 1e-8),
     ("CCC", 256, 40, 35, 60, 120, 1e-10),
     ("CCC", 512, 64, 48, 80, 170, 1e-8),
+    # cores of >= 96 columns take the multi-workgroup Jacobi (rows <= 256, <= 512, <= 768 per tile)
+    ("CCC", 512, 170, 150, 170, 170, 1e-8),
+    ("CCC", 1024, 341, 300, 341, 341, 1e-10),
 ]
 
 
@@ -149,6 +152,24 @@ def test_oracle_parity_fp64(ctx, case):
     tail = np.sqrt(np.sum(so[C.rank:] ** 2)) if C.rank < len(so) else 0.0
     assert np.linalg.norm(got - dense_ref) <= tail * 1.01 + 1e-12 * np.linalg.norm(dense_ref) * m
     # U has orthonormal columns
+    u, _ = C.factors()
+    assert np.linalg.norm(u.T @ u - np.eye(C.rank)) < 1e-12 * m
+
+
+@pytest.mark.parametrize("m,r", [(512, 170), (1024, 300)])
+def test_wide_core_multi_workgroup_jacobi(ctx, m, r):
+    """Slowly decaying spectra keep the whole core (kt = K = 2r columns): the plain path's
+    multi-workgroup Jacobi must give the oracle's singular values and rank."""
+    print(f"[case] wide core m={m} r={r}", flush=True)
+    rng = np.random.default_rng(77 + m)
+    au, av, bu, bv, cu, cv = _random_update(rng, m, m, r, r, r, decay=0.985)
+    C, oC, sg, so, dense_ref = _run_pair(ctx, "CCC", au, av, bu, bv, cu, cv, 1.0, 1.0, dict(accuracy=1e-8),
+                                         max_rank=2 * r)
+    assert len(sg) == len(so)
+    assert np.max(np.abs(sg - so)) <= 1e3 * EPS * so[0]
+    assert rank_tie(so, oC.rank, C.rank, 1e-8), (oC.rank, C.rank)
+    if C.rank == oC.rank:
+        assert rel_fro(C.to_dense(), oC.to_dense()) < 1e-10
     u, _ = C.factors()
     assert np.linalg.norm(u.T @ u - np.eye(C.rank)) < 1e-12 * m
 
