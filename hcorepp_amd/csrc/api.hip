@@ -252,7 +252,8 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
             const WsLayout L = ws_layout<T>(C->rows, C->cols, kmax, C->max_rank);
             need += aligned(sizeof(T) * L.total) + aligned(sizeof(int64_t) * 8) +
                     aligned(sizeof(double) * (size_t) std::max<int64_t>(kmax, 1)) +
-                    aligned(sizeof(int32_t) * JACOBI_MAX_SWEEPS);
+                    aligned(sizeof(int32_t) * JACOBI_MAX_SWEEPS) +
+                    aligned(sizeof(T) * ((size_t) std::max(C->rows, C->cols) * 16 + 1024));
         }
     }
     if ((st = arena_begin(ctx, need + 4096))) return st;
@@ -357,7 +358,9 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
         for (int i = 0; i < 16; ++i) ctx->last_info[i] = info[i];
     }
     int32_t *jflags = (int32_t *) arena_take(ctx, sizeof(int32_t) * JACOBI_MAX_SWEEPS);
-    HIP_TRY(launch_recomp_pre<T>(s, d, K, kLdsBytes, pb, jflags, reinterpret_cast<int32_t *>(ctx->pinned + HDR_WORDS)));
+    T *vpan = (T *) arena_take(ctx, sizeof(T) * ((size_t) std::max(m, n) * 16 + 1024));
+    HIP_TRY(launch_recomp_pre<T>(s, d, K, kLdsBytes, pb, jflags, reinterpret_cast<int32_t *>(ctx->pinned + HDR_WORDS),
+                                 vpan));
     // The plain form reads the new rank back with one stream synchronisation,
     // as the reference does (src/kernels/cuda/CudaKernels.cu:656-697).
     const int64_t *hdr = reinterpret_cast<const int64_t *>(d.ws + L.hdr);

@@ -51,6 +51,137 @@ __global__ void HCAMD_KERNEL k_svd(RecompDesc<T> d) {
 }
 
 // ---------------------------------------------------------------------------
+// Multi-workgroup blocked QR of the plain form: one launch factors a panel (one workgroup,
+// window pivoting included for the QR of Y) and writes its unit-lower reflectors to `vpan`;
+// the next applies the block reflector to the trailing columns, a few column batches per
+// workgroup over the whole GPU.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct QrOperand {
+    T *A;
+    int m, ncols;
+    int64_t lda;
+    T *tau, *tsave;
+    bool pivot;
+};
+
+template <typename T, int PB>
+__device__ QrOperand<T> qr_operand(const RecompDesc<T> &d, const Ws<T> &w, const Dims &z, int which) {
+    if (which == 0) return QrOperand<T>{w.us, d.m, z.K, d.m, w.tau_u, w.t_u, false};
+    return QrOperand<T>{w.y, d.n, z.mu, d.n, w.tau_y, w.t_y, window_pivot_fits(d.n, z.mu, PB)};
+}
+
+// Squared norms of the columns of the QR operand `which` (all rows), one wave per column; the
+// window pivoting of the first panel reads them.
+template <typename T, int PB>
+__global__ void HCAMD_KERNEL k_qr_colnorms(RecompDesc<T> d, int which, T *cn) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const QrOperand<T> q = qr_operand<T, PB>(d, w, z, which);
+    const int l = lane_id();
+    for (int c = blockIdx.x * NWAVES + wave_id(); c < q.ncols; c += gridDim.x * NWAVES) {
+        const T *col = q.A + (int64_t) c * q.lda;
+        T sum = T(0);
+        for (int r = l; r < q.m; r += WAVE) sum += col[r] * col[r];
+        sum = wave_sum(sum);
+        if (l == 0) cn[c] = sum;
+    }
+}
+
+template <typename T, int PB>
+__global__ void HCAMD_KERNEL k_qr_panel(RecompDesc<T> d, int which, int j0, T *vpan, const T *cn) {
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const QrOperand<T> q = qr_operand<T, PB>(d, w, z, which);
+    if (which == 1 && j0 == 0) {
+        for (int c = threadIdx.x; c < q.ncols; c += NT) w.piv[c] = c;
+        __syncthreads();
+    }
+    wg_geqrf_panel<T, PB>(q.m, q.ncols, q.A, q.lda, q.tau, q.tsave, lds, q.pivot ? w.piv : nullptr, w.hdr, j0,
+                          q.pivot ? cn : nullptr);
+    const int rows = q.m - j0;
+    for (int e = threadIdx.x; e < rows * PB; e += NT) vpan[e] = lds[e];
+}
+
+template <typename T, int PB, int RPT>
+__global__ void HCAMD_KERNEL k_qr_update(RecompDesc<T> d, int which, int j0, int cols_per_wg, const T *vpan,
+                                         T *cn) {
+    __shared__ T red[(NWAVES + 2) * WAVE];
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const QrOperand<T> q = qr_operand<T, PB>(d, w, z, which);
+    const int nsteps = q.m < q.ncols ? q.m : q.ncols;
+    const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
+    const int c0 = j0 + jb + (int) blockIdx.x * cols_per_wg;
+    const int c1 = (c0 + cols_per_wg) < q.ncols ? (c0 + cols_per_wg) : q.ncols;
+    if (c0 >= c1) return;
+    apply_wy<T, PB, RPT>(vpan, q.tsave + (int64_t) (j0 / PB) * PB * PB, q.m - j0, q.A + j0, q.lda, c0, c1, true, red);
+    if (q.pivot) {
+        // norms over rows [j0 + jb, m) of the updated columns, for the next panel's pivoting
+        __syncthreads();
+        const int l = lane_id();
+        for (int c = c0 + wave_id(); c < c1; c += NWAVES) {
+            const T *col = q.A + (int64_t) c * q.lda;
+            T sum = T(0);
+            for (int r = j0 + jb + l; r < q.m; r += WAVE) sum += col[r] * col[r];
+            sum = wave_sum(sum);
+            if (l == 0) cn[c] = sum;
+        }
+    }
+}
+
+// Trailing update with the row-chunk count of the panel's rows (rows <= 512 for PB 16, <= 1024
+// for PB 8, <= 2048 for PB 4, as pick_pb guarantees).
+template <typename T, int PB>
+static void launch_qr_update(hipStream_t s, const RecompDesc<T> &d, int which, int j0, int rows, int grid, int cpw,
+                             const T *vpan, T *cn) {
+    const int chunks = (rows + WAVE * NWAVES - 1) / (WAVE * NWAVES);
+    if (chunks <= 1)
+        hipLaunchKernelGGL((k_qr_update<T, PB, 1>), dim3(grid), dim3(NT), 0, s, d, which, j0, cpw, vpan, cn);
+    else if (chunks <= 2)
+        hipLaunchKernelGGL((k_qr_update<T, PB, 2>), dim3(grid), dim3(NT), 0, s, d, which, j0, cpw, vpan, cn);
+    else if constexpr (PB <= 8) {
+        if (chunks <= 4)
+            hipLaunchKernelGGL((k_qr_update<T, PB, 4>), dim3(grid), dim3(NT), 0, s, d, which, j0, cpw, vpan, cn);
+        else if constexpr (PB <= 4)
+            hipLaunchKernelGGL((k_qr_update<T, PB, 8>), dim3(grid), dim3(NT), 0, s, d, which, j0, cpw, vpan, cn);
+    }
+}
+
+template <typename T, int PB>
+static hipError_t launch_qr_mw_pb(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int which, int m,
+                                  int ncols, T *vpan) {
+    constexpr int NCB = WAVE / PB;
+    const int nsteps = m < ncols ? m : ncols;
+    T *cn = vpan + (int64_t) (d.m > d.n ? d.m : d.n) * 16;  // after the largest panel
+    if (which == 1)
+        hipLaunchKernelGGL((k_qr_colnorms<T, PB>), dim3((ncols + NWAVES - 1) / NWAVES), dim3(NT), 0, s, d, which, cn);
+    for (int j0 = 0; j0 < nsteps; j0 += PB) {
+        hipLaunchKernelGGL((k_qr_panel<T, PB>), dim3(1), dim3(NT), lds_bytes, s, d, which, j0, vpan, (const T *) cn);
+        const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
+        const int trail = ncols - (j0 + jb);
+        if (trail > 0) {
+            int cpw = NCB;
+            while ((trail + cpw - 1) / cpw > 512) cpw += NCB;
+            launch_qr_update<T, PB>(s, d, which, j0, m - j0, (trail + cpw - 1) / cpw, cpw, vpan, cn);
+        }
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_qr_mw(hipStream_t s, const RecompDesc<T> &d, int64_t lds_bytes, int pb, int which, int m,
+                               int ncols, T *vpan) {
+    switch (pb) {
+        case 16: return launch_qr_mw_pb<T, 16>(s, d, lds_bytes, which, m, ncols, vpan);
+        case 8: return launch_qr_mw_pb<T, 8>(s, d, lds_bytes, which, m, ncols, vpan);
+        default: return launch_qr_mw_pb<T, 4>(s, d, lds_bytes, which, m, ncols, vpan);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Multi-workgroup core SVD of the plain (one update per launch sequence) form.  A large core
 // (hundreds of columns) is the whole cost of an update, so its one-sided block Jacobi is spread
 // over the GPU: each launch is one step of the round-robin ordering of column blocks, one
@@ -354,16 +485,18 @@ static int clamp_grid(int64_t work) {
 
 template <typename T>
 hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb,
-                             int32_t *jflags, int32_t *host_flag) {
+                             int32_t *jflags, int32_t *host_flag, T *vpan) {
     const int64_t m = d.m, n = d.n;
     const int64_t mu = m < K ? m : K;
     const int64_t kk = n < mu ? n : mu;
     hipLaunchKernelGGL(k_stack<T>, dim3(clamp_grid(m * K)), dim3(NT), 0, s, d);
-    hipError_t e = launch_qr<T>(s, d, lds_bytes, pb, 0);
+    hipError_t e = (vpan && K >= MW_MIN_QR_COLS) ? launch_qr_mw<T>(s, d, lds_bytes, pb, 0, (int) m, (int) K, vpan)
+                                                 : launch_qr<T>(s, d, lds_bytes, pb, 0);
     if (e != hipSuccess) return e;
     const int ytiles = (int) (((n + GT_M - 1) / GT_M) * ((mu + GT_N - 1) / GT_N));
     hipLaunchKernelGGL(k_y<T>, dim3(ytiles), dim3(NT), 0, s, d);
-    e = launch_qr<T>(s, d, lds_bytes, pb, 1);
+    e = (vpan && mu >= MW_MIN_QR_COLS) ? launch_qr_mw<T>(s, d, lds_bytes, pb, 1, (int) n, (int) mu, vpan)
+                                       : launch_qr<T>(s, d, lds_bytes, pb, 1);
     if (e != hipSuccess) return e;
     if (jflags && host_flag && kk >= MW_MIN_COLS && mu <= JACOBI_ROWS_MAX)
         return launch_core_svd_mw<T>(s, d, (int) mu, (int) kk, jflags, host_flag);
@@ -418,6 +551,7 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
     // Allow the dynamic-LDS kernels to use more than the default 64 KB.
     const void *fns[] = {
             (const void *) k_qr<T, 16>, (const void *) k_qr<T, 8>, (const void *) k_qr<T, 4>,
+            (const void *) k_qr_panel<T, 16>, (const void *) k_qr_panel<T, 8>, (const void *) k_qr_panel<T, 4>,
             (const void *) k_recon<T, 16>, (const void *) k_recon<T, 8>, (const void *) k_recon<T, 4>,
             (const void *) k_batched<T, 16>, (const void *) k_batched<T, 8>, (const void *) k_batched<T, 4>,
     };
@@ -432,7 +566,7 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
     template hipError_t launch_gemm<T>(hipStream_t, int, int, int, T, const T *, int64_t, bool, const T *,      \
                                        int64_t, bool, T, T *, int64_t);                                          \
     template hipError_t launch_recomp_pre<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int, int32_t *, \
-                                             int32_t *);                                                         \
+                                             int32_t *, T *);                                                    \
     template hipError_t launch_recomp_post<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);        \
     template hipError_t launch_readjust_ddc<T>(hipStream_t, const T *, int64_t, int64_t, T *);                   \
     template hipError_t launch_batched<T>(hipStream_t, const BatchEntry<T> *, const int32_t *, int64_t,          \

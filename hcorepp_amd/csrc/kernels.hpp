@@ -54,11 +54,14 @@ hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A,
                        int64_t ldb, bool tb, T beta, T *C, int64_t ldc);
 // Cores of at least MW_MIN_COLS columns run their Jacobi over many workgroups when `jflags`
 // (JACOBI_MAX_SWEEPS device ints) and `host_flag` (one pinned int) are given; that path
-// synchronises the stream between sweep chunks.
+// synchronises the stream between sweep chunks.  QRs of at least MW_MIN_QR_COLS columns run
+// panel by panel with multi-workgroup trailing updates when `vpan` (max(m, n) * 16 + 1024
+// elements of device scratch: the panel, then column norms) is given.
 constexpr int MW_MIN_COLS = 96;
+constexpr int MW_MIN_QR_COLS = 128;
 template <typename T>
 hipError_t launch_recomp_pre(hipStream_t s, const RecompDesc<T> &d, int64_t K, int64_t lds_bytes, int pb,
-                             int32_t *jflags = nullptr, int32_t *host_flag = nullptr);
+                             int32_t *jflags = nullptr, int32_t *host_flag = nullptr, T *vpan = nullptr);
 template <typename T>
 hipError_t launch_recomp_post(hipStream_t s, const RecompDesc<T> &d, int64_t r, int64_t lds_bytes, int pb);
 hipError_t launch_set_i64(hipStream_t s, int64_t *p, int64_t v);

@@ -352,27 +352,33 @@ __host__ __device__ inline int64_t qr_lds_elems(int64_t m, int pb) {
     return m * pb + 2 * pb * pb + (NWAVES + 2) * WAVE + 64;
 }
 
-// Window pivoting of a blocked QR at panel start j0: the jb trailing columns of largest
+// Window pivoting of a blocked QR at panel start j0 (column norms from `cn` when given): the jb trailing columns of largest
 // norm over rows [j0, m) are moved, in decreasing norm order, to positions [j0, j0 + jb)
 // by jb column swaps (whole columns, so the R rows above j0 move with them); piv records
 // the permutation.  Uses the panel buffer P as scratch (needs m * PB >= 2 * (ncols - j0) + 2 * PB).
 template <typename T, int PB>
-__device__ void window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, int32_t *piv, T *P) {
+__device__ void window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, int32_t *piv, T *P,
+                             const T *cn = nullptr) {
     const int nt = ncols - j0;
     const int l = lane_id();
     T *nrm = P;
     int *at = reinterpret_cast<int *>(nrm + nt);  // at[p]: column (by starting position) now at p
     int *where = at + nt;                         // where[c]: current position of column c
     int *sel = where + nt;                        // sel[t]: t-th largest column; sel[PB + t]: swap source
-    for (int c = wave_id(); c < nt; c += NWAVES) {
-        const T *col = A + (int64_t) (j0 + c) * lda;
-        T sum = T(0);
-        for (int r = j0 + l; r < m; r += WAVE) {
-            const T v = col[r];
-            sum += v * v;
+    if (cn) {
+        // squared norms over rows [j0, m) by current column position, left by the previous trailing update
+        for (int c = threadIdx.x; c < nt; c += NT) nrm[c] = cn[j0 + c];
+    } else {
+        for (int c = wave_id(); c < nt; c += NWAVES) {
+            const T *col = A + (int64_t) (j0 + c) * lda;
+            T sum = T(0);
+            for (int r = j0 + l; r < m; r += WAVE) {
+                const T v = col[r];
+                sum += v * v;
+            }
+            sum = wave_sum(sum);
+            if (l == 0) nrm[c] = sum;
         }
-        sum = wave_sum(sum);
-        if (l == 0) nrm[c] = sum;
     }
     for (int c = threadIdx.x; c < nt; c += NT) {
         at[c] = c;
@@ -432,9 +438,14 @@ __host__ __device__ inline bool window_pivot_fits(int64_t m, int64_t ncols, int 
 // With piv non-null, each panel starts with window pivoting (window_pivot) and
 // piv[0..ncols) receives the column permutation: A P = Q R.
 // ---------------------------------------------------------------------------
+
+// One panel of the blocked QR below: window pivoting (when piv is given), Householder
+// factorisation of columns [j0, j0 + jb) over rows [j0, m), write-back, and the compact-WY T
+// (saved to tsave[j0 / PB] when tsave is given).  Leaves the panel's unit-lower reflectors in
+// LDS P (rows x PB, rows = m - j0) and T in LDS Tm; returns jb.
 template <typename T, int PB>
-__device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr,
-                         int64_t *hdr = nullptr) {
+__device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv,
+                              int64_t *hdr, int j0, const T *cn = nullptr) {
     (void) hdr;
     T *P = lds;
     T *Tm = P + (int64_t) m * PB;
@@ -442,104 +453,119 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
     T *red = gram + PB * PB;  // (NWAVES + 2) * WAVE
     const int nsteps = m < ncols ? m : ncols;
     const int l = lane_id();
+    (void) l;
+    const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
+    const int rows = m - j0;
+    HCAMD_SUB_BEGIN(t_pan);
+    HCAMD_SUB_BEGIN(t_piv);
+    if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P, cn);
+    HCAMD_SUB_END(t_piv, hdr, 5);
+    for (int e = threadIdx.x; e < rows * PB; e += NT) {
+        const int r = e % rows, c = e / rows;
+        P[e] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
+    }
+    __syncthreads();
+    for (int jj = 0; jj < jb; ++jj) {
+        T part = T(0);
+        for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
+            const T v = P[r + jj * rows];
+            part += v * v;
+        }
+        const T xnorm2 = wg_sum(part, red);
+        const T alpha = P[jj + jj * rows];
+        T tau_j, beta, scal;
+        if (xnorm2 == T(0)) {
+            tau_j = T(0);
+            beta = alpha;
+            scal = T(0);
+        } else {
+            beta = -dsign(sqrt(alpha * alpha + xnorm2), alpha);
+            tau_j = (beta - alpha) / beta;
+            scal = T(1) / (alpha - beta);
+        }
+        __syncthreads();
+        if (tau_j != T(0)) {
+            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) P[r + jj * rows] *= scal;
+        }
+        if (threadIdx.x == 0) {
+            P[jj + jj * rows] = beta;
+            tau[j0 + jj] = tau_j;
+        }
+        __syncthreads();
+        if (tau_j != T(0) && jj + 1 < jb) {
+            // w_k = x_k[jj] + sum_{r>jj} v_r x_k[r] for panel columns k in (jj, jb)
+            T acc[PB];
+#pragma unroll
+            for (int k = 0; k < PB; ++k) acc[k] = T(0);
+            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
+                const T v = P[r + jj * rows];
+#pragma unroll
+                for (int k = 0; k < PB; ++k)
+                    if (k > jj && k < jb) acc[k] += v * P[r + k * rows];
+            }
+            const T s = wave_multi_sum_fast<T, PB>(acc);
+            const int idx = wave_multi_sum_index<PB>(l);
+            if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
+            __syncthreads();
+            if (threadIdx.x < PB) {
+                const int k = threadIdx.x;
+                if (k > jj && k < jb) {
+                    T wsum = P[jj + k * rows];
+                    for (int wv = 0; wv < NWAVES; ++wv) wsum += red[wv * PB + k];
+                    gram[k] = tau_j * wsum;
+                }
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < rows * PB; e += NT) {
+                const int r = e % rows, k = e / rows;
+                if (k > jj && k < jb && r >= jj) {
+                    const T v = (r == jj) ? T(1) : P[r + jj * rows];
+                    P[e] -= gram[k] * v;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    HCAMD_SUB_END(t_pan, hdr, 0);
+    // write the factored panel back (R on/above the diagonal, reflectors below)
+    for (int e = threadIdx.x; e < rows * jb; e += NT) {
+        const int r = e % rows, c = e / rows;
+        A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[e];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * PB; e += NT) {
+        const int r = e % rows, c = e / rows;
+        if (c < jb) {
+            if (r == c) P[e] = T(1);
+            else if (r < c) P[e] = T(0);
+        }
+    }
+    __syncthreads();
+    HCAMD_SUB_BEGIN(t_bt);
+    build_t<T, PB>(P, rows, tau + j0, jb, Tm, gram);
+    if (tsave) {
+        for (int e = threadIdx.x; e < PB * PB; e += NT) tsave[(int64_t) (j0 / PB) * PB * PB + e] = Tm[e];
+    }
+    HCAMD_SUB_END(t_bt, hdr, 1);
+    return jb;
+}
+
+// The whole blocked QR in one workgroup: panels and trailing updates in turn.
+template <typename T, int PB>
+__device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr,
+                         int64_t *hdr = nullptr) {
+    T *P = lds;
+    T *Tm = P + (int64_t) m * PB;
+    T *red = Tm + 2 * PB * PB;
+    const int nsteps = m < ncols ? m : ncols;
     if (piv) {
         for (int c = threadIdx.x; c < ncols; c += NT) piv[c] = c;
         __syncthreads();
     }
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
-        const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
-        const int rows = m - j0;
-        HCAMD_SUB_BEGIN(t_pan);
-        if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P);
-        for (int e = threadIdx.x; e < rows * PB; e += NT) {
-            const int r = e % rows, c = e / rows;
-            P[e] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
-        }
-        __syncthreads();
-        for (int jj = 0; jj < jb; ++jj) {
-            T part = T(0);
-            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-                const T v = P[r + jj * rows];
-                part += v * v;
-            }
-            const T xnorm2 = wg_sum(part, red);
-            const T alpha = P[jj + jj * rows];
-            T tau_j, beta, scal;
-            if (xnorm2 == T(0)) {
-                tau_j = T(0);
-                beta = alpha;
-                scal = T(0);
-            } else {
-                beta = -dsign(sqrt(alpha * alpha + xnorm2), alpha);
-                tau_j = (beta - alpha) / beta;
-                scal = T(1) / (alpha - beta);
-            }
-            __syncthreads();
-            if (tau_j != T(0)) {
-                for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) P[r + jj * rows] *= scal;
-            }
-            if (threadIdx.x == 0) {
-                P[jj + jj * rows] = beta;
-                tau[j0 + jj] = tau_j;
-            }
-            __syncthreads();
-            if (tau_j != T(0) && jj + 1 < jb) {
-                // w_k = x_k[jj] + sum_{r>jj} v_r x_k[r] for panel columns k in (jj, jb)
-                T acc[PB];
-#pragma unroll
-                for (int k = 0; k < PB; ++k) acc[k] = T(0);
-                for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-                    const T v = P[r + jj * rows];
-#pragma unroll
-                    for (int k = 0; k < PB; ++k)
-                        if (k > jj && k < jb) acc[k] += v * P[r + k * rows];
-                }
-                const T s = wave_multi_sum_fast<T, PB>(acc);
-                const int idx = wave_multi_sum_index<PB>(l);
-                if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
-                __syncthreads();
-                if (threadIdx.x < PB) {
-                    const int k = threadIdx.x;
-                    if (k > jj && k < jb) {
-                        T wsum = P[jj + k * rows];
-                        for (int wv = 0; wv < NWAVES; ++wv) wsum += red[wv * PB + k];
-                        gram[k] = tau_j * wsum;
-                    }
-                }
-                __syncthreads();
-                for (int e = threadIdx.x; e < rows * PB; e += NT) {
-                    const int r = e % rows, k = e / rows;
-                    if (k > jj && k < jb && r >= jj) {
-                        const T v = (r == jj) ? T(1) : P[r + jj * rows];
-                        P[e] -= gram[k] * v;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        HCAMD_SUB_END(t_pan, hdr, 0);
-        // write the factored panel back (R on/above the diagonal, reflectors below)
-        for (int e = threadIdx.x; e < rows * jb; e += NT) {
-            const int r = e % rows, c = e / rows;
-            A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[e];
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < rows * PB; e += NT) {
-            const int r = e % rows, c = e / rows;
-            if (c < jb) {
-                if (r == c) P[e] = T(1);
-                else if (r < c) P[e] = T(0);
-            }
-        }
-        __syncthreads();
-        HCAMD_SUB_BEGIN(t_bt);
-        build_t<T, PB>(P, rows, tau + j0, jb, Tm, gram);
-        if (tsave) {
-            for (int e = threadIdx.x; e < PB * PB; e += NT) tsave[(int64_t) (j0 / PB) * PB * PB + e] = Tm[e];
-        }
-        HCAMD_SUB_END(t_bt, hdr, 1);
+        const int jb = wg_geqrf_panel<T, PB>(m, ncols, A, lda, tau, tsave, lds, piv, hdr, j0);
         HCAMD_SUB_BEGIN(t_up);
-        if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, rows, A + j0, lda, j0 + jb, ncols, true, red);
+        if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, m - j0, A + j0, lda, j0 + jb, ncols, true, red);
         __syncthreads();
         HCAMD_SUB_END(t_up, hdr, 2);
     }
