@@ -155,6 +155,90 @@ def plain_tlr_leg(H, ctx, n_ctiles):
     return out
 
 
+CHOL_N, CHOL_NB, CHOL_ELL, CHOL_ACC, CHOL_SEED, CHOL_NUGGET = 16384, 512, 0.1, 1e-8, 20261205, 1e-2
+
+
+def cholesky_leg(H, ctx, forms):
+    """TLR Cholesky A = L L^T of a 32 x 32-tile SPD matrix (nb = 512): exp(-r/0.1) + 1e-2 I over
+    16384 points in the unit cube, off-diagonal tiles compressed at 1e-8 (maxRank 170).  Each
+    form factors a fresh device copy; the step loop (potrf, trsm, syrk, transposes, trailing
+    updates) is timed end to end.  Backward error ||A - L L^T||_F / ||A||_F against the dense
+    matrix and against the compressed input, with L reassembled densely."""
+    import ctypes as C
+    from hcorepp_amd.cholesky import TLRCholesky
+    from hcorepp_amd.workloads import TLRMatrix, prefetch_tiles
+    t0 = time.time()
+    M = TLRMatrix(CHOL_N, CHOL_NB, 3, CHOL_ELL, CHOL_ACC, CHOL_SEED, nugget=CHOL_NUGGET)
+    T = M.T
+    prefetch_tiles(M, [(i, j) for i in range(T) for j in range(i + 1)], min(16, os.cpu_count() or 8))
+    ranks0 = [M.tile(i, j)[1].shape[1] for i in range(T) for j in range(i)]
+    log(f"[bench] TLR Cholesky {T}x{T} tiles nb={CHOL_NB}: tiles built in {time.time() - t0:.1f}s, input ranks "
+        f"{min(ranks0)}-{max(ranks0)} (mean {np.mean(ranks0):.1f})")
+    n, nb = CHOL_N, CHOL_NB
+    A = M.dense_block_range(0, n)  # symmetric: its C-order buffer is also its column-major buffer
+    nA = float(np.linalg.norm(A))
+    At = np.zeros_like(A)  # the compressed input, assembled
+    for i in range(T):
+        for j in range(i + 1):
+            t = M.tile(i, j)
+            blk = t[1] if t[0] == "D" else t[1] @ t[2]
+            At[i * nb:(i + 1) * nb, j * nb:(j + 1) * nb] = blk
+            if i != j:
+                At[j * nb:(j + 1) * nb, i * nb:(i + 1) * nb] = blk.T
+    nAt = float(np.linalg.norm(At))
+    input_err = float(np.linalg.norm(A - At)) / nA
+    dA, dAt = H.DeviceBuffer.from_numpy(A.reshape(-1)), H.DeviceBuffer.from_numpy(At.reshape(-1))
+    dR, dL = H.DeviceBuffer(A.nbytes), H.DeviceBuffer(A.nbytes)
+    del At
+
+    def residual(src, dLbuf, norm):
+        """||S - L L^T||_F / norm on the device; dLbuf holds L^T column-major (L in C order)."""
+        dR.copy_from(src, A.nbytes)
+        H._check(H.lib().hcorepp_kernel_gemm_d(ctx.h, b"T", b"N", n, n, n, -1.0, dLbuf.ptr, n, dLbuf.ptr, n, 1.0,
+                                               dR.ptr, n))
+        ctx.sync()
+        return float(np.linalg.norm(dR.download(np.float64))) / norm
+    params = H.CompressionParameters(CHOL_ACC)
+    n_upd = sum((T - 1 - k) * (T - 2 - k) // 2 for k in range(T))
+    out = {"workload": f"TLR Cholesky, {T}x{T} tiles, nb={CHOL_NB}: exp(-r/{CHOL_ELL}) + {CHOL_NUGGET} I over "
+                       f"{CHOL_N} points in the unit cube (seed {CHOL_SEED}), acc {CHOL_ACC}, maxRank {CHOL_NB // 3}; "
+                       f"{T} potrf, {T * (T - 1) // 2} trsm, {T * (T - 1) // 2} syrk, {n_upd} trailing updates",
+           "input_compression_rel_err": input_err, "forms": {}}
+    for name in forms:
+        F = TLRCholesky(ctx, M)
+        ctx.sync()
+        tt = time.perf_counter()
+        if name == "plain_fp64":
+            F.factor_plain(params, progress=lambda k: log(f"[bench] TLR Cholesky plain_fp64: step {k}/{T} done "
+                                                          f"after {time.perf_counter() - tt:.1f}s") if k % 8 == 0
+                           else None)
+        elif name == "batched_fp64":
+            F.factor_batched(params)
+        elif name == "batched_rule":
+            F.factor_batched(params, tol=LEARNED_TOL)
+        elif name == "batched_mixed":
+            F.factor_batched(params, precision=H.PREC_MIXED)
+        ctx.sync()
+        dt = time.perf_counter() - tt
+        dL.upload(np.ascontiguousarray(F.dense_factor()).reshape(-1))
+        be = residual(dA, dL, nA)
+        be_t = residual(dAt, dL, nAt)
+        rk = list(F.ranks().values())
+        out["forms"][name] = {"seconds": round(dt, 3), "trailing_updates_per_s": round(n_upd / dt, 1),
+                              "backward_error_vs_A": be, "backward_error_vs_compressed_input": be_t,
+                              "final_ranks_min_max_mean": [int(min(rk)), int(max(rk)), round(float(np.mean(rk)), 1)]}
+        log(f"[bench] TLR Cholesky {name}: {dt:.2f}s ({n_upd / dt:.0f} trailing updates/s); backward error "
+            f"{be:.2e} vs A, {be_t:.2e} vs the compressed input; ranks {min(rk)}-{max(rk)}")
+        del F
+    if "plain_fp64" in out["forms"]:
+        base = out["forms"]["plain_fp64"]["seconds"]
+        for name, f in out["forms"].items():
+            f["speedup_vs_plain_fp64"] = round(base / f["seconds"], 2)
+    for b in (dA, dAt, dR, dL):
+        b.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +255,8 @@ def main():
                     help="comma-separated forms to measure (profiling runs pick one)")
     ap.add_argument("--plain-tlr-tiles", type=int, default=2,
                     help="C tiles of the nb=1024 TLR covariance GEMM timed in the plain form (0 = skip)")
+    ap.add_argument("--cholesky-forms", default="batched_fp64,batched_rule,batched_mixed,plain_fp64",
+                    help="TLR Cholesky forms to time (empty = skip the Cholesky leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -293,6 +379,12 @@ def main():
     log(f"[bench] plain FP64 (sample {ns}): {ns / tp:.1f} updates/s; work-weighted estimate over the batch "
         f"{plain_full_est:.1f} updates/s")
     plain_tlr = plain_tlr_leg(H, ctx, args.plain_tlr_tiles) if args.plain_tlr_tiles > 0 else None
+    chol_forms = [f for f in args.cholesky_forms.split(",") if f]
+    if chol_forms:
+        batch.close()  # free the batch workspace before the Cholesky leg
+        chol = cholesky_leg(H, ctx, chol_forms)
+    else:
+        chol = None
 
     # ---- CPU baseline (the oracle) on the whole batch; its outputs also check every form
     from oracle import oracle as O
@@ -379,6 +471,7 @@ def main():
         "rel_err_checked_updates": len(orc_dense),
         "forms": {k: form_json(k, f) for k, f in forms.items()},
         "plain_tlr_nb1024": plain_tlr,
+        "tlr_cholesky_nb512": chol,
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved_tflops, 4),

@@ -54,7 +54,7 @@ class _CParams(C.Structure):
 EXPORTED = [
     "hcorepp_ctx_create", "hcorepp_ctx_destroy", "hcorepp_ctx_sync", "hcorepp_ctx_stream",
     "hcorepp_ctx_arena_bytes", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
-    "hcorepp_memcpy_async", "hcorepp_memset",
+    "hcorepp_memcpy_async", "hcorepp_memset", "hcorepp_kernel_gemm_d",
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
@@ -104,6 +104,7 @@ def lib():
     L.hcorepp_memcpy.argtypes = [P, P, P, C.c_size_t]
     L.hcorepp_memset.argtypes = [P, P, C.c_int, C.c_size_t]
     L.hcorepp_memcpy_async.argtypes = [P, P, P, C.c_size_t]
+    L.hcorepp_kernel_gemm_d.argtypes = [P, C.c_char, C.c_char, I64, I64, I64, D, P, I64, P, I64, D, P, I64]
     L.hcorepp_malloc.argtypes = [C.c_size_t, C.POINTER(P)]
     L.hcorepp_free.argtypes = [P]
     L.hcorepp_ctx_event_record.argtypes = [P, C.c_int]

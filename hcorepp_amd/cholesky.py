@@ -70,8 +70,9 @@ class TLRCholesky:
         _check(lib().hcorepp_transpose_batch_d(self.ctx.h, len(src), sa, da))
         _sync_back(dst, da)
 
-    def factor_plain(self, params: CompressionParameters):
-        """One call per tile operation; trailing updates through the plain HCore.gemm."""
+    def factor_plain(self, params: CompressionParameters, progress=None):
+        """One call per tile operation; trailing updates through the plain HCore.gemm.
+        `progress(k)` is called after step k."""
         for k in range(self.T):
             self._potrf(k)
             for i in range(k + 1, self.T):
@@ -83,6 +84,8 @@ class TLRCholesky:
             for j in range(k + 1, self.T):
                 for i in range(j + 1, self.T):
                     HCore.gemm(-1.0, self.L[(i, k)], self.LT[(j, k)], 1.0, self.L[(i, j)], self.ctx, params)
+            if progress is not None:
+                progress(k)
         self.ctx.sync()
 
     def factor_batched(self, params: CompressionParameters, precision=PREC_FP64, tol=0.0):

@@ -517,6 +517,22 @@ hcorepp_status hcorepp_memcpy_async(hcorepp_ctx *ctx, void *dst, const void *src
     return HCOREPP_OK;
 }
 
+hcorepp_status hcorepp_kernel_gemm_d(hcorepp_ctx *ctx, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                                     double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+                                     double beta, double *C, int64_t ldc) {
+    if (!ctx || m < 0 || n < 0 || k < 0 || (m * n > 0 && !C) || (m * n * k > 0 && (!A || !B)))
+        return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    if ((transa != 'N' && transa != 'T') || (transb != 'N' && transb != 'T'))
+        return fail(HCOREPP_ERR_INVALID, "trans must be 'N' or 'T'");
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return fail(HCOREPP_ERR_UNSUPPORTED, "dimension too large");
+    const int64_t ar = transa == 'N' ? m : k, br = transb == 'N' ? k : n;
+    if (ldc < std::max<int64_t>(m, 1) || lda < std::max<int64_t>(ar, 1) || ldb < std::max<int64_t>(br, 1))
+        return fail(HCOREPP_ERR_INVALID, "leading dimension too small");
+    HIP_TRY(launch_gemm<double>(ctx->stream, (int) m, (int) n, (int) k, alpha, A, lda, transa == 'T', B, ldb,
+                                transb == 'T', beta, C, ldc));
+    return HCOREPP_OK;
+}
+
 hcorepp_status hcorepp_memset(hcorepp_ctx *ctx, void *dst, int value, size_t bytes) {
     if (!bytes) return HCOREPP_OK;
     if (ctx) HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));

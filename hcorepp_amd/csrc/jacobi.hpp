@@ -478,7 +478,7 @@ __device__ __forceinline__ void coop_apply_v(T *V, int ncols, int64_t ldv, int c
 // One tile of the multi-workgroup Jacobi; `lds` holds coop_rec_lds_elems<T, TW>() elements.
 // Returns whether any pair was rotated (uniform over the workgroup).
 template <typename T, int RPL, int TW>
-__device__ int coop_jacobi_tile_rec(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ,
+__device__ __forceinline__ int coop_jacobi_tile_rec(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ,
                                     bool full, T tol, T floor2, T *lds) {
     constexpr int B = TW / 2;
     T *red = lds;

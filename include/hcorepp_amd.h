@@ -101,6 +101,13 @@ hcorepp_status hcorepp_free(void *ptr);
 hcorepp_status hcorepp_memcpy(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Stream-ordered copy on the context stream, no synchronisation. */
 hcorepp_status hcorepp_memcpy_async(hcorepp_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* Dense column-major C = alpha op(A) op(B) + beta C on device pointers, ordered on the context
+ * stream (no synchronisation); trans 'N' or 'T'.  Replaces HCoreKernels<T>::Gemm
+ * (reference include/hcorepp/kernels/kernels.hpp:27-29), ColMajor only. */
+hcorepp_status hcorepp_kernel_gemm_d(hcorepp_ctx *ctx, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                                     double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+                                     double beta, double *C, int64_t ldc);
 hcorepp_status hcorepp_memset(hcorepp_ctx *ctx, void *dst, int value, size_t bytes);
 
 /* Stream-ordered timing on the context stream: record event `slot` (0..15) and read the

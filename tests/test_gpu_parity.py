@@ -273,3 +273,23 @@ def test_batched_mixed_precision_error_is_fp32_level(ctx):
         b.close()
     assert res[H.PREC_FP64] < 1e-12
     assert 1e-12 < res[H.PREC_MIXED] < 1e-5
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_kernel_gemm_matches_numpy(ctx, ta, tb):
+    """hcorepp_kernel_gemm_d (HCoreKernels<T>::Gemm) on ragged sizes, all transposes."""
+    H = _H()
+    rng = np.random.default_rng(5)
+    m, n, k = 131, 77, 45
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    Cm = rng.standard_normal((m, n))
+    dA = H.DeviceBuffer.from_numpy(np.asfortranarray(A).reshape(-1, order="F"))
+    dB = H.DeviceBuffer.from_numpy(np.asfortranarray(B).reshape(-1, order="F"))
+    dC = H.DeviceBuffer.from_numpy(np.asfortranarray(Cm).reshape(-1, order="F"))
+    H._check(H.lib().hcorepp_kernel_gemm_d(ctx.h, ta.encode(), tb.encode(), m, n, k, -0.5, dA.ptr, A.shape[0],
+                                           dB.ptr, B.shape[0], 2.0, dC.ptr, m))
+    ctx.sync()
+    got = dC.download(np.float64).reshape(m, n, order="F")
+    ref = -0.5 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) + 2.0 * Cm
+    assert rel_fro(got, ref) < 1e-14
