@@ -5,63 +5,132 @@
 //   syrk   A_ii <- A_ii - L_ik L_ik^T = A_ii - U (V V^T) U^T for a compressed L_ik (dense A_ii)
 //   transpose  compressed B = A^T: U_B = V_A^T, V_B = U_A^T (the B operand of the trailing
 //              updates A_ij <- A_ij - L_ik L_jk^T, which then run as ordinary CCC recompressions)
-// Each kernel handles a batch of independent tiles, one workgroup per tile.
+// The trsm, syrk and transpose kernels handle a batch of independent tiles, one workgroup per
+// tile; potrf is blocked over several launches.
 #include "kernels.hpp"
 #include "cholesky.hpp"
 
 namespace hcamd {
 
-// Right-looking unblocked Cholesky of one n x n tile in global memory (the tile of an nb = 512
-// TLR matrix is L2-resident).  info[0] receives the first non-positive pivot (1-based), 0 if none.
-__global__ void HCAMD_KERNEL k_potrf(double *A, int n, int64_t lda, int *info) {
-    __shared__ double s_piv;
-    __shared__ int s_bad;
-    if (threadIdx.x == 0) s_bad = 0;
-    __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        if (threadIdx.x == 0) {
-            const double d = A[j + (int64_t) j * lda];
-            if (!(d > 0.0)) {
-                if (!s_bad) s_bad = j + 1;
-                s_piv = 1.0;
-            } else {
-                s_piv = sqrt(d);
+constexpr int CB = 32;   // column block of the blocked potrf
+constexpr int RB = 16;   // column block of the row-wise triangular solves
+constexpr int ROWS_TRSM_LDS = RB * (RB + 1) + RB * NT;
+
+// X <- X L^{-T} for X (rows x n, column k at X + k * ldx) and lower-triangular L (n x n, ld ldl),
+// i.e. the solve X L^T = B by rows.  A thread owns a row: blocks of RB columns are accumulated in
+// registers from the earlier (already solved) columns, with L staged through LDS in RB x RB
+// pieces; the diagonal block is then solved by forward substitution in LDS (the thread's RB
+// values in column-interleaved slots, so that loops stay rolled).  `lds` holds ROWS_TRSM_LDS
+// doubles.  Every thread of the workgroup must call it (it synchronises).
+__device__ __forceinline__ void rows_trsm_lt(double *X, int rows, int n, int64_t ldx, const double *L, int64_t ldl,
+                                             double *lds) {
+    double *lt = lds;                   // RB x (RB + 1)
+    double *xs = lds + RB * (RB + 1);   // RB x NT
+    const int tid = (int) threadIdx.x;
+    for (int rbase = 0; rbase < rows; rbase += NT) {
+        const int a = rbase + tid;
+        const bool act = a < rows;
+        for (int j0 = 0; j0 < n; j0 += RB) {
+            const int jb = (n - j0) < RB ? (n - j0) : RB;
+            double acc[RB];
+#pragma unroll
+            for (int c = 0; c < RB; ++c) acc[c] = (act && c < jb) ? X[a + (int64_t) (j0 + c) * ldx] : 0.0;
+            for (int k0 = 0; k0 < j0; k0 += RB) {
+                __syncthreads();
+                for (int e = tid; e < RB * RB; e += NT) {
+                    const int c = e % RB, kk = e / RB;
+                    lt[c * (RB + 1) + kk] = c < jb ? L[(j0 + c) + (int64_t) (k0 + kk) * ldl] : 0.0;
+                }
+                __syncthreads();
+                if (act) {
+                    for (int kk = 0; kk < RB; ++kk) {
+                        const double x = X[a + (int64_t) (k0 + kk) * ldx];
+#pragma unroll
+                        for (int c = 0; c < RB; ++c) acc[c] -= x * lt[c * (RB + 1) + kk];
+                    }
+                }
             }
-            A[j + (int64_t) j * lda] = s_piv;
+            __syncthreads();
+            for (int e = tid; e < RB * RB; e += NT) {
+                const int c = e % RB, cc = e / RB;
+                lt[c * (RB + 1) + cc] = (c < jb && cc < jb) ? L[(j0 + c) + (int64_t) (j0 + cc) * ldl] : 1.0;
+            }
+#pragma unroll
+            for (int c = 0; c < RB; ++c) xs[c * NT + tid] = acc[c];
+            __syncthreads();
+            if (act) {
+                for (int c = 0; c < jb; ++c) {
+                    double x = xs[c * NT + tid];
+                    for (int cc = 0; cc < c; ++cc) x -= xs[cc * NT + tid] * lt[c * (RB + 1) + cc];
+                    x /= lt[c * (RB + 1) + c];
+                    xs[c * NT + tid] = x;
+                    X[a + (int64_t) (j0 + c) * ldx] = x;
+                }
+            }
+        }
+    }
+}
+
+// Diagonal block of the blocked potrf: unblocked Cholesky of A[j0:j0+jb, j0:j0+jb] in LDS.
+// info[0] receives the first non-positive pivot (1-based, whole-tile numbering) unless already set;
+// such a pivot is replaced by 1 so that the factorisation runs to the end.
+__global__ void HCAMD_KERNEL k_potrf_diag(double *A, int64_t lda, int j0, int jb, int *info) {
+    __shared__ double d[CB * (CB + 1)];
+    for (int e = threadIdx.x; e < jb * jb; e += NT) {
+        const int i = e % jb, c = e / jb;
+        d[i + c * (CB + 1)] = A[(j0 + i) + (int64_t) (j0 + c) * lda];
+    }
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+        if (threadIdx.x == 0) {
+            const double v = d[j + j * (CB + 1)];
+            double p = 1.0;
+            if (v > 0.0) p = sqrt(v);
+            else if (info[0] == 0) info[0] = j0 + j + 1;
+            d[j + j * (CB + 1)] = p;
         }
         __syncthreads();
-        const double piv = s_piv;
-        for (int i = j + 1 + threadIdx.x; i < n; i += NT) A[i + (int64_t) j * lda] /= piv;
+        const double p = d[j + j * (CB + 1)];
+        for (int i = j + 1 + threadIdx.x; i < jb; i += NT) d[i + j * (CB + 1)] /= p;
         __syncthreads();
-        // trailing update of the lower triangle: A[i, c] -= A[i, j] A[c, j] for j < c <= i
-        const int m = n - j - 1;
-        for (int64_t e = threadIdx.x; e < (int64_t) m * m; e += NT) {
-            const int i = j + 1 + (int) (e % m), c = j + 1 + (int) (e / m);
-            if (i >= c) A[i + (int64_t) c * lda] -= A[i + (int64_t) j * lda] * A[c + (int64_t) j * lda];
+        for (int e = threadIdx.x; e < jb * jb; e += NT) {
+            const int i = e % jb, c = e / jb;
+            if (c > j && i >= c) d[i + c * (CB + 1)] -= d[i + j * (CB + 1)] * d[c + j * (CB + 1)];
         }
         __syncthreads();
     }
-    for (int64_t e = threadIdx.x; e < (int64_t) n * n; e += NT) {
+    for (int e = threadIdx.x; e < jb * jb; e += NT) {
+        const int i = e % jb, c = e / jb;
+        if (i >= c) A[(j0 + i) + (int64_t) (j0 + c) * lda] = d[i + c * (CB + 1)];
+    }
+}
+
+// Sub-diagonal panel of the blocked potrf: A[j0+jb:n, j0:j0+jb] <- (that) L_jj^{-T}, rows dealt to
+// workgroups NT at a time.
+__global__ void HCAMD_KERNEL k_potrf_panel(double *A, int n, int64_t lda, int j0, int jb) {
+    __shared__ double lds[ROWS_TRSM_LDS];
+    const int r0 = j0 + jb + (int) blockIdx.x * NT;
+    const int rows = (n - r0) < NT ? (n - r0) : NT;
+    if (rows <= 0) return;
+    rows_trsm_lt(A + r0 + (int64_t) j0 * lda, rows, jb, lda, A + j0 + (int64_t) j0 * lda, lda, lds);
+}
+
+__global__ void HCAMD_KERNEL k_zero_upper(double *A, int n, int64_t lda) {
+    for (int64_t e = (int64_t) blockIdx.x * NT + threadIdx.x; e < (int64_t) n * n; e += (int64_t) gridDim.x * NT) {
         const int i = (int) (e % n), c = (int) (e / n);
         if (i < c) A[i + (int64_t) c * lda] = 0.0;
     }
-    if (threadIdx.x == 0) *info = s_bad;
 }
 
-// V (r x n, ld r) <- V L^{-T} for lower-triangular L (n x n, ld ldl): forward substitution
-// X L^T = V row by row; lane a owns row a of V.
+__global__ void k_zero_int(int *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// V (r x n, ld r) <- V L^{-T} for lower-triangular L (n x n, ld ldl): one workgroup per tile.
 __global__ void HCAMD_KERNEL k_trsm_v(const CholTrsm *jobs) {
+    __shared__ double lds[ROWS_TRSM_LDS];
     const CholTrsm jb = jobs[blockIdx.x];
-    const int r = (int) jb.rank, n = (int) jb.n;
-    double *V = jb.v;
-    const double *L = jb.l;
-    for (int a = threadIdx.x; a < r; a += NT) {
-        for (int j = 0; j < n; ++j) {
-            double acc = V[a + (int64_t) j * r];
-            for (int k = 0; k < j; ++k) acc -= V[a + (int64_t) k * r] * L[j + (int64_t) k * jb.ldl];
-            V[a + (int64_t) j * r] = acc / L[j + (int64_t) j * jb.ldl];
-        }
-    }
+    rows_trsm_lt(jb.v, (int) jb.rank, (int) jb.n, jb.rank, jb.l, jb.ldl, lds);
 }
 
 // A (n x n dense) <- A - U (V V^T) U^T, with scratch W (r x r) and T (n x r); the strict upper
@@ -100,8 +169,27 @@ __global__ void HCAMD_KERNEL k_transpose_factors(const CholTrans *jobs) {
     }
 }
 
+// Blocked right-looking potrf: per block column, the diagonal block (one workgroup), the panel
+// below it (row solves over the GPU), and the trailing update A22 -= L21 L21^T as a dense GEMM
+// (both triangles; the strict upper part is cleared at the end).
 hipError_t launch_potrf(hipStream_t s, double *A, int n, int64_t lda, int *info) {
-    hipLaunchKernelGGL(k_potrf, dim3(1), dim3(NT), 0, s, A, n, lda, info);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, info);
+    for (int j0 = 0; j0 < n; j0 += CB) {
+        const int jb = (n - j0) < CB ? (n - j0) : CB;
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(NT), 0, s, A, lda, j0, jb, info);
+        const int rest = n - j0 - jb;
+        if (rest > 0) {
+            hipLaunchKernelGGL(k_potrf_panel, dim3((rest + NT - 1) / NT), dim3(NT), 0, s, A, n, lda, j0, jb);
+            double *L21 = A + (j0 + jb) + (int64_t) j0 * lda;
+            hipError_t e = launch_gemm<double>(s, rest, rest, jb, -1.0, L21, lda, false, L21, lda, true, 1.0,
+                                               A + (j0 + jb) + (int64_t) (j0 + jb) * lda, lda);
+            if (e != hipSuccess) return e;
+        }
+    }
+    int64_t work = (int64_t) n * n;
+    int grid = (int) ((work + NT - 1) / NT);
+    if (grid > 1024) grid = 1024;
+    hipLaunchKernelGGL(k_zero_upper, dim3(grid > 0 ? grid : 1), dim3(NT), 0, s, A, n, lda);
     return hipGetLastError();
 }
 hipError_t launch_trsm_v(hipStream_t s, const CholTrsm *jobs, int n) {

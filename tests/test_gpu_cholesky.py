@@ -73,12 +73,14 @@ def test_syrk_known_answer(ctx):
     assert np.array_equal(np.triu(out, 1), np.zeros((4, 4)))
 
 
-def test_tile_ops_random_match_oracle(ctx):
-    """potrf / trsm / syrk / transpose on random 128-tiles of several ranks, batched."""
+@pytest.mark.parametrize("nb", [128, 200, 512])
+def test_tile_ops_random_match_oracle(ctx, nb):
+    """potrf / trsm / syrk / transpose on random tiles of several ranks, batched (tile sizes on and
+    off the 32-column block of the blocked potrf and the 16-column block of the row solves)."""
     H, HC, OC = _mods()
     import ctypes as C
-    rng = np.random.default_rng(11)
-    nb = 128
+    print(f"[case] tile ops nb={nb}", flush=True)
+    rng = np.random.default_rng(11 + nb)
     G = rng.standard_normal((nb, nb))
     S = G @ G.T + nb * np.eye(nb)
     D = H.DenseTile(S)
@@ -87,15 +89,15 @@ def test_tile_ops_random_match_oracle(ctx):
     Lg = D.to_numpy()
     Lo = OC.potrf(S)
     assert rel_fro(Lg, Lo) < 1e-13
-    ranks = [0, 1, 7, 33, 64]
+    ranks = [0, 1, 7, 33, 64, 300] if nb == 512 else [0, 1, 7, 33, 64]
     us = [rng.standard_normal((nb, r)) for r in ranks]
     vs = [rng.standard_normal((r, nb)) for r in ranks]
-    Bs = [H.CompressedTile(u, v, max_rank=80) for u, v in zip(us, vs)]
+    Bs = [H.CompressedTile(u, v, max_rank=max(ranks) + 16) for u, v in zip(us, vs)]
     H._check(H.lib().hcorepp_trsm_batch_d(ctx.h, len(Bs), C.byref(D.c()), (H._Tile * len(Bs))(*[b.c() for b in Bs])))
     Cs = [H.DenseTile(S) for _ in ranks]
     H._check(H.lib().hcorepp_syrk_batch_d(ctx.h, len(Bs), (H._Tile * len(Bs))(*[b.c() for b in Bs]),
                                           (H._Tile * len(Cs))(*[c.c() for c in Cs])))
-    Ts = [H.CompressedTile(np.zeros((nb, 0)), np.zeros((0, nb)), max_rank=80) for _ in ranks]
+    Ts = [H.CompressedTile(np.zeros((nb, 0)), np.zeros((0, nb)), max_rank=max(ranks) + 16) for _ in ranks]
     src = (H._Tile * len(Bs))(*[b.c() for b in Bs])
     dst = (H._Tile * len(Ts))(*[t.c() for t in Ts])
     H._check(H.lib().hcorepp_transpose_batch_d(ctx.h, len(Bs), src, dst))
