@@ -54,7 +54,7 @@ class _CParams(C.Structure):
 EXPORTED = [
     "hcorepp_ctx_create", "hcorepp_ctx_destroy", "hcorepp_ctx_sync", "hcorepp_ctx_stream",
     "hcorepp_ctx_arena_bytes", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
-    "hcorepp_memcpy_async", "hcorepp_memset", "hcorepp_kernel_gemm_d",
+    "hcorepp_memcpy_async", "hcorepp_memset", "hcorepp_kernel_gemm_d", "hcorepp_compress_d", "hcorepp_compress_s",
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
@@ -105,6 +105,8 @@ def lib():
     L.hcorepp_memset.argtypes = [P, P, C.c_int, C.c_size_t]
     L.hcorepp_memcpy_async.argtypes = [P, P, P, C.c_size_t]
     L.hcorepp_kernel_gemm_d.argtypes = [P, C.c_char, C.c_char, I64, I64, I64, D, P, I64, P, I64, D, P, I64]
+    L.hcorepp_compress_d.argtypes = [P, I64, I64, P, I64, C.POINTER(_CParams), C.POINTER(_Tile)]
+    L.hcorepp_compress_s.argtypes = [P, I64, I64, P, I64, C.POINTER(_CParams), C.POINTER(_Tile)]
     L.hcorepp_malloc.argtypes = [C.c_size_t, C.POINTER(P)]
     L.hcorepp_free.argtypes = [P]
     L.hcorepp_ctx_event_record.argtypes = [P, C.c_int]
@@ -368,6 +370,25 @@ class CompressedTile(_TileBase):
         else:
             self.buf = arena.take(host.nbytes)
             self.buf.upload(host)
+
+    @classmethod
+    def compress(cls, ctx: "RunContext", a, params: "CompressionParameters | None" = None, max_rank=None,
+                 dtype=np.float64) -> "CompressedTile":
+        """Compression constructor on the device (CompressedTile(rows, cols, data, ld, params, ...),
+        reference src/operators/concrete/Compressed.cpp:75-146): max_rank defaults to the
+        reference's max(min(rows, cols) / 3, 1)."""
+        a = np.asfortranarray(np.asarray(a, dtype=dtype))
+        m, n = a.shape
+        mr = max(min(m, n) // 3, 1) if max_rank is None else int(max_rank)
+        t = cls(np.zeros((m, 0), dtype=dtype), np.zeros((0, n), dtype=dtype), max_rank=mr, dtype=dtype)
+        src = DeviceBuffer.from_numpy(a.reshape(-1, order="F"))
+        d = t.c()
+        p = (params or CompressionParameters(1e-9)).c()
+        f = lib().hcorepp_compress_s if dtype == np.float32 else lib().hcorepp_compress_d
+        _check(f(ctx.h, m, n, src.ptr, m, C.byref(p), C.byref(d)))
+        t.update_from(d)
+        src.free()
+        return t
 
     def factors(self):
         h = self.buf.download(self.dtype)

@@ -193,6 +193,21 @@ hcorepp_status validate_tile(const hcorepp_tile *t, const char *name) {
     return HCOREPP_OK;
 }
 
+// Arena bytes of one plain recompression (workspace, rank word, sigma, Jacobi flags, QR panel scratch).
+template <typename T>
+size_t recompress_arena_bytes(int64_t m, int64_t n, int64_t kmax, int64_t max_rank) {
+    const WsLayout L = ws_layout<T>(m, n, kmax, max_rank);
+    return aligned(sizeof(T) * L.total) + aligned(sizeof(int64_t) * 8) +
+           aligned(sizeof(double) * (size_t) std::max<int64_t>(kmax, 1)) +
+           aligned(sizeof(int32_t) * JACOBI_MAX_SWEEPS) +
+           aligned(sizeof(T) * ((size_t) std::max(m, n) * 16 + 1024));
+}
+
+template <typename T>
+hcorepp_status recompress_plain(hcorepp_ctx *ctx, T alpha, const Holder<T> &ad, const Holder<T> &bd, T beta,
+                                hcorepp_tile *C, const hcorepp_cparams &p, uint64_t *flops, double *sigma_out,
+                                int64_t *n_sigma);
+
 template <typename T>
 hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char opA, const hcorepp_tile *B,
                           char opB, T beta, hcorepp_tile *C, const hcorepp_cparams *params, int cholesky,
@@ -249,11 +264,7 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
         if (C->type == HCOREPP_COMPRESSED && !ddc) {
             const int64_t arank = a2[ai].cols;
             const int64_t kmax = arank + C->rank;
-            const WsLayout L = ws_layout<T>(C->rows, C->cols, kmax, C->max_rank);
-            need += aligned(sizeof(T) * L.total) + aligned(sizeof(int64_t) * 8) +
-                    aligned(sizeof(double) * (size_t) std::max<int64_t>(kmax, 1)) +
-                    aligned(sizeof(int32_t) * JACOBI_MAX_SWEEPS) +
-                    aligned(sizeof(T) * ((size_t) std::max(C->rows, C->cols) * 16 + 1024));
+            need += recompress_arena_bytes<T>(C->rows, C->cols, kmax, C->max_rank);
         }
     }
     if ((st = arena_begin(ctx, need + 4096))) return st;
@@ -319,6 +330,16 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
         return HCOREPP_OK;
     }
     // ---- compressed C: recompression (CompressedTile<T>::Gemm) ----
+    return recompress_plain<T>(ctx, alpha, ad, bd, beta, C, p, flops, sigma_out, n_sigma);
+}
+
+// Recompression of C <- alpha * ad * bd + beta * C (compressed C, ad: m x arank, bd: arank x n) in the
+// plain form; the context arena must already hold recompress_arena_bytes for this update.
+template <typename T>
+hcorepp_status recompress_plain(hcorepp_ctx *ctx, T alpha, const Holder<T> &ad, const Holder<T> &bd, T beta,
+                                hcorepp_tile *C, const hcorepp_cparams &p, uint64_t *flops, double *sigma_out,
+                                int64_t *n_sigma) {
+    hipStream_t s = ctx->stream;
     const int64_t m = C->rows, n = C->cols, arank = ad.cols, rc = C->rank;
     const int64_t K = arank + rc;
     if (ad.rows != m || bd.cols != n || bd.rows != arank)
@@ -383,6 +404,33 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
     C->rank = r;
     C->ld = m;
     return HCOREPP_OK;
+}
+
+// Compression constructor on the device: the truncated SVD of the dense tile A is the
+// recompression of A * I from an empty C, so it reuses the plain path (QR, window-pivoted QR,
+// Jacobi core SVD, the rank rule, reconstruction into the caller's tile buffer).
+template <typename T>
+hcorepp_status compress_plain(hcorepp_ctx *ctx, int64_t m, int64_t n, const T *A, int64_t lda,
+                              const hcorepp_cparams *params, hcorepp_tile *out) {
+    hcorepp_status st;
+    if (!ctx || !out || (m * n > 0 && !A)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    if (m < 1 || n < 1 || lda < m) return fail(HCOREPP_ERR_INVALID, "bad tile shape %lldx%lld (ld %lld)",
+                                               (long long) m, (long long) n, (long long) lda);
+    if ((st = validate_tile<T>(out, "out"))) return st;
+    if (out->type != HCOREPP_COMPRESSED || out->rows != m || out->cols != n || out->max_rank < 1 || !out->data)
+        return fail(HCOREPP_ERR_INVALID, "out must be a compressed %lldx%lld tile with max_rank >= 1",
+                    (long long) m, (long long) n);
+    const hcorepp_cparams p = params ? *params : hcorepp_cparams{1e-9, 0, 1, 0, HCOREPP_SVD_GESDD, 0};
+    if ((st = ensure_lds_limits())) return st;
+    if ((st = arena_begin(ctx, recompress_arena_bytes<T>(m, n, n, out->max_rank) +
+                                   aligned(sizeof(T) * (size_t) (n * n)) + 4096)))
+        return st;
+    T *I = (T *) arena_take(ctx, sizeof(T) * (size_t) (n * n));
+    HIP_TRY(launch_fill_identity<T>(ctx->stream, I, n));
+    const Holder<T> ad{m, n, lda, const_cast<T *>(A)}, bd{n, n, n, I};
+    out->rank = 0;
+    out->ld = m;
+    return recompress_plain<T>(ctx, T(1), ad, bd, T(0), out, p, nullptr, nullptr, nullptr);
 }
 
 }  // namespace
@@ -552,6 +600,16 @@ hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile 
                               const hcorepp_cparams *params, int cholesky, uint64_t *flops, double *sigma_out,
                               int64_t *n_sigma) {
     return gemm_plain<float>(ctx, alpha, A, opA, B, opB, beta, C, params, cholesky, flops, sigma_out, n_sigma);
+}
+
+hcorepp_status hcorepp_compress_d(hcorepp_ctx *ctx, int64_t rows, int64_t cols, const double *A, int64_t lda,
+                                  const hcorepp_cparams *params, hcorepp_tile *out) {
+    return compress_plain<double>(ctx, rows, cols, A, lda, params, out);
+}
+
+hcorepp_status hcorepp_compress_s(hcorepp_ctx *ctx, int64_t rows, int64_t cols, const float *A, int64_t lda,
+                                  const hcorepp_cparams *params, hcorepp_tile *out) {
+    return compress_plain<float>(ctx, rows, cols, A, lda, params, out);
 }
 
 size_t hcorepp_gemm_pool_size_d(const hcorepp_tile *A, const hcorepp_tile *B, const hcorepp_tile *C,

@@ -326,6 +326,20 @@ __global__ void HCAMD_KERNEL k_readjust_ddc(const T *W, int64_t m, int64_t n, T 
     }
 }
 
+template <typename T>
+__global__ void HCAMD_KERNEL k_fill_identity(T *I, int64_t n) {
+    for (int64_t e = (int64_t) blockIdx.x * NT + threadIdx.x; e < n * n; e += (int64_t) gridDim.x * NT)
+        I[e] = ((e % n) == (e / n)) ? T(1) : T(0);
+}
+
+template <typename T>
+hipError_t launch_fill_identity(hipStream_t s, T *I, int64_t n) {
+    int64_t grid = (n * n + NT - 1) / NT;
+    if (grid > 1024) grid = 1024;
+    hipLaunchKernelGGL(k_fill_identity<T>, dim3((unsigned) (grid > 0 ? grid : 1)), dim3(NT), 0, s, I, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Batched persistent kernel.
 // ---------------------------------------------------------------------------
@@ -569,6 +583,7 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
                                              int32_t *, T *);                                                    \
     template hipError_t launch_recomp_post<T>(hipStream_t, const RecompDesc<T> &, int64_t, int64_t, int);        \
     template hipError_t launch_readjust_ddc<T>(hipStream_t, const T *, int64_t, int64_t, T *);                   \
+    template hipError_t launch_fill_identity<T>(hipStream_t, T *, int64_t);                                      \
     template hipError_t launch_batched<T>(hipStream_t, const BatchEntry<T> *, const int32_t *, int64_t,          \
                                           BatchCounters *, int, int64_t, int, int);                              \
     template hipError_t set_kernel_lds_limits<T>(int64_t);

@@ -67,6 +67,8 @@ hipError_t launch_recomp_post(hipStream_t s, const RecompDesc<T> &d, int64_t r, 
 hipError_t launch_set_i64(hipStream_t s, int64_t *p, int64_t v);
 template <typename T>
 hipError_t launch_readjust_ddc(hipStream_t s, const T *W, int64_t m, int64_t n, T *cdata);
+template <typename T>
+hipError_t launch_fill_identity(hipStream_t s, T *I, int64_t n);
 hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c);
 template <typename T>
 hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,

@@ -1,16 +1,28 @@
 // Drop-in for hcorepp::operators::CompressedTile<T>
 // (reference include/hcorepp/operators/concrete/Compressed.hpp:29-258,
-// src/operators/concrete/Compressed.cpp:19-73): one device buffer
+// src/operators/concrete/Compressed.cpp:19-146): one device buffer
 // [U rows x maxRank (ld rows) | V maxRank x cols (ld rank)], maxRank = rank at construction
 // as in the reference, so the rank of a C tile never grows past its initial rank.
 #pragma once
 
 #include <algorithm>
 
+#include <hcorepp/operators/helpers/CompressionParameters.hpp>
 #include <hcorepp/operators/interface/Tile.hpp>
 
 namespace hcorepp {
 namespace operators {
+
+namespace detail {
+inline hcorepp_status compress(hcorepp_ctx *c, int64_t m, int64_t n, const double *a, int64_t lda,
+                               const hcorepp_cparams *p, hcorepp_tile *t) {
+    return hcorepp_compress_d(c, m, n, a, lda, p, t);
+}
+inline hcorepp_status compress(hcorepp_ctx *c, int64_t m, int64_t n, const float *a, int64_t lda,
+                               const hcorepp_cparams *p, hcorepp_tile *t) {
+    return hcorepp_compress_s(c, m, n, a, lda, p, t);
+}
+}  // namespace detail
 
 template <typename T>
 class CompressedTile : public Tile<T> {
@@ -37,6 +49,27 @@ public:
                    size_t aMaxRank, const kernels::RunContext &aContext) {
         Init(aNumOfRows, aNumOfCols, aLeadingDim, aRank, std::max(aRank, aMaxRank));
         if (apData) CopyIn(apData, apData + aNumOfRows * aRank, aContext);
+    }
+    // Compression constructor (Compressed.cpp:75-146): truncated SVD of the dense tile at apData
+    // (host or device memory, ld aLeadingDim) on the device; maxRank = max(min(rows, cols) / 3, 1)
+    // as in the reference; without data the rank is maxRank and the buffer is left unset.
+    CompressedTile(size_t aNumOfRows, size_t aNumOfCols, T *apData, size_t aLeadingDim,
+                   const CompressionParameters &aParameters, blas::Layout aLayout,
+                   const kernels::RunContext &aContext) {
+        (void) aLayout;
+        const size_t mr = std::max<size_t>(std::min(aNumOfRows, aNumOfCols) / 3, 1);
+        Init(aNumOfRows, aNumOfCols, aNumOfRows, mr, mr);
+        if (!apData) return;
+        const size_t bytes = aLeadingDim * aNumOfCols * sizeof(T);
+        void *dev = nullptr;
+        kernels::ThrowIfError(hcorepp_malloc(bytes, &dev));
+        hcorepp_status st = hcorepp_memcpy(aContext.Handle(), dev, apData, bytes);
+        const hcorepp_cparams p = aParameters.ToC();
+        if (st == HCOREPP_OK)
+            st = detail::compress(aContext.Handle(), (int64_t) aNumOfRows, (int64_t) aNumOfCols, (const T *) dev,
+                                  (int64_t) aLeadingDim, &p, &this->mDesc);
+        hcorepp_free(dev);
+        kernels::ThrowIfError(st);
     }
     // Wraps caller-owned device memory described by a TileMetadata (PackTile, Compressed.cpp:781-799).
     CompressedTile(const TileMetadata &aMeta, T *apDeviceData) {

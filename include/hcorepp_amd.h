@@ -124,8 +124,8 @@ hcorepp_status hcorepp_ctx_event_elapsed(hcorepp_ctx *ctx, int slot0, int slot1,
  *    reference's CompressedTile::Gemm ignores the ops, Compressed.cpp:331-380); other ops
  *    return HCOREPP_ERR_UNSUPPORTED.  Dense C supports 'N' and 'T'.
  *  - cholesky != 0 (the reference's aCholesky operand re-plumbing, HCore.cpp:160-174, used
- *    only by its TLR Cholesky driver) returns HCOREPP_ERR_UNSUPPORTED: the Cholesky tile
- *    operations are not part of this library.
+ *    only by its TLR Cholesky driver) returns HCOREPP_ERR_UNSUPPORTED: the TLR Cholesky here
+ *    passes the transposed panel tile explicitly (hcorepp_transpose_batch_d below).
  *  - Plain form: the recompression runs as a chain of launches on the context stream and the
  *    new rank is read back with one stream synchronisation, as the reference does
  *    (src/kernels/cuda/CudaKernels.cu:656-697).  C->rank is updated on return.
@@ -141,6 +141,20 @@ hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile 
                               const hcorepp_tile *B, char opB, float beta, hcorepp_tile *C,
                               const hcorepp_cparams *params, int cholesky, uint64_t *flops,
                               double *sigma_out, int64_t *n_sigma);
+
+/* Compression constructor on the device: replaces CompressedTile<T>(rows, cols, apData, ld,
+ * CompressionParameters, layout, context) (reference src/operators/concrete/Compressed.cpp:75-146).
+ * A: device pointer, rows x cols column-major (ld lda).  out: compressed tile descriptor with
+ * rows/cols equal to A's, a device buffer of capacity [U rows x max_rank | V max_rank x cols]
+ * and max_rank >= 1 (the reference uses max(min(rows, cols) / 3, 1)).  Rank rule as the
+ * reference: fixed_rank if set (clamped to min(rows, cols)), else the first singular value
+ * below accuracy (relative to sigma_0 with truncated_svd), at least 1, at most max_rank;
+ * U orthonormal, V = Sigma V^T.  out->rank and out->ld are set; synchronises the stream once.
+ * min(rows, cols) <= 768. */
+hcorepp_status hcorepp_compress_d(hcorepp_ctx *ctx, int64_t rows, int64_t cols, const double *A, int64_t lda,
+                                  const hcorepp_cparams *params, hcorepp_tile *out);
+hcorepp_status hcorepp_compress_s(hcorepp_ctx *ctx, int64_t rows, int64_t cols, const float *A, int64_t lda,
+                                  const hcorepp_cparams *params, hcorepp_tile *out);
 
 /* Workspace (elements of T) one update needs: reference HCore<T>::CalculateMemoryPoolSize
  * (src/api/HCore.cpp:417-480).  The engine's layout differs, so the number is this engine's. */

@@ -57,6 +57,24 @@ int main(int argc, char **argv) {
     delete packed;
     delete md.first;
     delete mdd.first;
-    std::printf("dropin_compile: max abs err %.3e flops %zu packer %s\n", err, flops, ok ? "ok" : "FAIL");
-    return (err < 1e-10 && ok) ? 0 : 1;
+    // Compression constructor (Compressed.cpp:75-146) on a rank-2 dense 12x9 tile from host memory
+    const size_t cm = 12, cn = 9;
+    std::vector<double> dense(cm * cn);
+    for (size_t j = 0; j < cn; ++j)
+        for (size_t i = 0; i < cm; ++i) dense[i + cm * j] = (1.0 + i) * (2.0 - 0.1 * j) + 0.5 * i * i * (j % 3);
+    CompressedTile<double> Z(cm, cn, dense.data(), cm, CompressionParameters(1e-10), blas::Layout::ColMajor, ctx);
+    std::vector<double> zu(cm * Z.GetTileRank()), zv(Z.GetTileRank() * cn);
+    ThrowIfError(hcorepp_memcpy(ctx.Handle(), zu.data(), Z.GetUMatrix(), zu.size() * sizeof(double)));
+    ThrowIfError(hcorepp_memcpy(ctx.Handle(), zv.data(), Z.GetVMatrix(), zv.size() * sizeof(double)));
+    double cerr = 0;
+    for (size_t j = 0; j < cn; ++j)
+        for (size_t i = 0; i < cm; ++i) {
+            double s = 0;
+            for (size_t k = 0; k < Z.GetTileRank(); ++k) s += zu[i + cm * k] * zv[k + Z.GetTileRank() * j];
+            cerr = std::max(cerr, std::abs(s - dense[i + cm * j]));
+        }
+    const bool cok = Z.GetTileRank() == 2 && Z.GetMaxRank() == 3 && cerr < 1e-9;
+    std::printf("dropin_compile: max abs err %.3e flops %zu packer %s compress rank %zu err %.3e\n", err, flops,
+                ok ? "ok" : "FAIL", Z.GetTileRank(), cerr);
+    return (err < 1e-10 && ok && cok) ? 0 : 1;
 }
