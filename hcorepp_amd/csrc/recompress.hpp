@@ -59,6 +59,7 @@ enum : int {
     HDR_STATUS = 4,  // status bits (STATUS_*)
     HDR_KT = 5,      // columns of the truncated core kept for the SVD (<= size_s)
     HDR_JFRO2 = 6,   // ||J||_F^2 of the truncated core as double bits (multi-workgroup Jacobi)
+    HDR_CQR = 7,     // 1: the core was QR-factored (J = Q2 R2) and the Jacobi ran on R2
     HDR_T0 = 8,      // 8 phase timestamps (builds with HCAMD_PHASE_TIMING only)
     HDR_SUB = 16,    // 8 sub-phase cycle accumulators (builds with HCAMD_PHASE_TIMING only)
     HDR_FEAT = 24,   // RANK_FEATURES float32 rank-predictor features (12 words)
@@ -96,7 +97,7 @@ struct RecompDesc {
 
 // Workspace layout (element offsets of T) for capacity kmax and tile m x n.
 struct WsLayout {
-    int64_t hdr, us, vs, y, tau_u, tau_y, t_u, t_y, j, v1, sig, perm, piv, vtmp, total;
+    int64_t hdr, us, vs, y, tau_u, tau_y, t_u, t_y, j, v1, sig, perm, piv, vtmp, j2, tau_j, t_j, total;
 };
 
 __host__ __device__ inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
@@ -122,6 +123,9 @@ __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax
     L.perm = o; o += align_up((kk * 4 + sizeof(T) - 1) / sizeof(T), 32);
     L.piv = o; o += align_up((kmax * 4 + sizeof(T) - 1) / sizeof(T), 32);
     L.vtmp = o; o += align_up(n * (max_rank > 0 ? max_rank : 1), 32);
+    L.j2 = o; o += align_up(kk * kk, 32);
+    L.tau_j = o; o += align_up(kk, 32);
+    L.t_j = o; o += align_up(kk * QR_PB_MAX + QR_PB_MAX * QR_PB_MAX, 32);
     L.total = o;
     return L;
 }
@@ -129,7 +133,7 @@ __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax
 template <typename T>
 struct Ws {
     int64_t *hdr;
-    T *us, *vs, *y, *tau_u, *tau_y, *t_u, *t_y, *j, *v1, *sig, *vtmp;
+    T *us, *vs, *y, *tau_u, *tau_y, *t_u, *t_y, *j, *v1, *sig, *vtmp, *j2, *tau_j, *t_j;
     int32_t *perm;
     int32_t *piv;  // column permutation of the pivoted QR of Y
 };
@@ -152,6 +156,9 @@ __device__ inline Ws<T> ws_of(const RecompDesc<T> &d) {
     w.perm = reinterpret_cast<int32_t *>(d.ws + L.perm);
     w.piv = reinterpret_cast<int32_t *>(d.ws + L.piv);
     w.vtmp = d.ws + L.vtmp;
+    w.j2 = d.ws + L.j2;
+    w.tau_j = d.ws + L.tau_j;
+    w.t_j = d.ws + L.t_j;
     return w;
 }
 
@@ -636,7 +643,7 @@ __device__ void ph_y(const RecompDesc<T> &d, int wg, int nwg, T *lds) {
 // of that level, and never below a requested fixed rank.
 // ph_core_size writes kt to the header and leaves in lds: rn[j] = ||R_Y[j:, :]||_F^2 (kk values)
 // followed by a lower bound of the rank threshold; ph_core_build then builds
-// J = R_Y[0:kt, :]^T (mu x kt, ld mu) and V1 = I (kt x kt).  `lds` holds >= kk + NWAVES values.
+// J = P R_Y[0:kt, :]^T (mu x kt, ld mu; P the pivoting of the QR of Y) and V1 = I (kt x kt).  `lds` holds >= kk + NWAVES values.
 template <typename T>
 __device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
@@ -678,6 +685,7 @@ __device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
         }
         if (kt < 1) kt = kk < 1 ? 0 : 1;
         w.hdr[HDR_KT] = kt;
+        w.hdr[HDR_CQR] = 0;
         rn[kk] = thr_lo;
     }
     __syncthreads();
@@ -688,9 +696,11 @@ __device__ void ph_core_build(const RecompDesc<T> &d) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int mu = z.mu, kt = z.kt, n = d.n;
+    // J[piv[i], j] = R_Y[j, i]: rows of J in the order of Q_U's basis (undoing the column pivoting
+    // of the QR of Y), so the left factor needs no permutation in the reconstruction
     for (int64_t e = threadIdx.x; e < (int64_t) mu * kt; e += NT) {
         const int64_t i = e % mu, j = e / mu;
-        w.j[e] = (i >= j) ? w.y[j + i * n] : T(0);
+        w.j[w.piv[i] + j * mu] = (i >= j) ? w.y[j + i * n] : T(0);
     }
     for (int64_t e = threadIdx.x; e < (int64_t) kt * kt; e += NT) {
         const int64_t i = e % kt, j = e / kt;
@@ -703,6 +713,28 @@ template <typename T>
 __device__ void ph_core(const RecompDesc<T> &d, T *lds) {
     ph_core_size<T>(d, lds);
     ph_core_build<T>(d);
+}
+
+// Second QR of a tall core (batched form): J (mu x kt) = Q2 R2, after which the Jacobi runs on the
+// kt x kt triangle R2 (copied to j2 with zeros below the diagonal) -- its rows, flops and passes
+// over memory shrink by mu / kt, and the extra QR preconditions the Jacobi further.  J keeps Q2's
+// reflectors (T factors in t_j) for the reconstruction.  Skipped for cores that are not at least
+// a quarter taller than wide.  `lds` holds qr_lds_elems(mu, PB) elements.
+constexpr int CORE_QR_MIN_COLS = 16;
+template <typename T, int PB>
+__device__ void ph_core_qr(const RecompDesc<T> &d, T *lds) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const int mu = z.mu, kt = z.kt;
+    if (kt < CORE_QR_MIN_COLS || 4 * mu < 5 * kt) return;
+    wg_geqrf<T, PB>(mu, kt, w.j, mu, w.tau_j, w.t_j, lds);
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < (int64_t) kt * kt; e += NT) {
+        const int64_t i = e % kt, c = e / kt;
+        w.j2[e] = (i <= c) ? w.j[i + c * mu] : T(0);
+    }
+    if (threadIdx.x == 0) w.hdr[HDR_CQR] = 1;
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -771,10 +803,14 @@ __device__ void ph_svd_finish(const RecompDesc<T> &d, int sweeps) {
     if (threadIdx.x == 0) s_nan = 0;
     __syncthreads();
     const int l = lane_id();
+    // the orthogonalised core: J (mu rows) or, after the core QR, R2 (kt rows)
+    const bool cqr = w.hdr[HDR_CQR] != 0;
+    const T *zc = cqr ? w.j2 : w.j;
+    const int zrows = cqr ? kt : mu;
     for (int j = wave_id(); j < kt; j += NWAVES) {
-        const T *zj = w.j + (int64_t) j * mu;
+        const T *zj = zc + (int64_t) j * zrows;
         T s = 0;
-        for (int r = l; r < mu; r += WAVE) s += zj[r] * zj[r];
+        for (int r = l; r < zrows; r += WAVE) s += zj[r] * zj[r];
         s = wave_sum(s);
         if (l == 0) {
             w.sig[j] = sqrt(s);
@@ -839,15 +875,20 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
         if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
     } else {
         HCAMD_SUB_BEGIN(t_jac);
-        sweeps = run_block_jacobi<T>(w.j, z.mu, z.kt, w.v1, lds);
+        if (w.hdr[HDR_CQR]) sweeps = run_block_jacobi<T>(w.j2, z.kt, z.kt, w.v1, lds);
+        else sweeps = run_block_jacobi<T>(w.j, z.mu, z.kt, w.v1, lds);
         HCAMD_SUB_END(t_jac, w.hdr, 3);
+#ifdef HCAMD_PHASE_TIMING
+        if (threadIdx.x == 0) w.hdr[HDR_SUB + 7] = sweeps + 1000 * z.kt;  // timing build: sweeps and core width
+#endif
     }
     ph_svd_finish<T>(d, sweeps);
 }
 
 // ---------------------------------------------------------------------------
 // Phase: reconstruction of columns [c0, c1) of the new rank.
-//   side 0: C_U[:, c] = Q_U [P z_perm(c) / sigma; 0]   (P: column pivoting of the QR of Y)
+//   side 0: C_U[:, c] = Q_U [z_perm(c) / sigma; 0]  (z: column of J = P R_Y^T after the Jacobi, or
+//           of Q2 R2 W after the core QR; P, the pivoting of the QR of Y, is applied when J is built)
 //   side 1: vtmp[:, c] = Q_Y [v1_perm(c) * sigma; 0]
 // ---------------------------------------------------------------------------
 template <typename T, int PB>
@@ -858,22 +899,37 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
     if (c1 > r) c1 = r;
     if (c0 >= c1) return;
     const int m = d.m, n = d.n, mu = z.mu, kk = z.kk, kt = z.kt;
-    if (side == 0) {
+    if (side == 0 && w.hdr[HDR_CQR]) {
         T *B = d.cdata;  // ld m
-        // row i < mu of the core's left factor belongs to row piv[i] of Q_U's basis
+        // left factor of the core J = Q2 R2: Q2 [Z2 / sigma; 0] in rows [0, mu), zero below
         for (int64_t e = threadIdx.x; e < (int64_t) m * (c1 - c0); e += NT) {
             const int i = (int) (e % m), c = c0 + (int) (e / m);
             T v = T(0);
-            int row = i;
-            if (i < mu) {
-                row = w.piv[i];
-                if (c < kt) {
-                    const int j = w.perm[c];
-                    const T sg = w.sig[j];
-                    v = (sg > T(0)) ? w.j[i + (int64_t) j * mu] / sg : T(0);
-                }
+            if (i < kt && c < kt) {
+                const int j = w.perm[c];
+                const T sg = w.sig[j];
+                v = (sg > T(0)) ? w.j2[i + (int64_t) j * kt] / sg : T(0);
             }
-            B[row + (int64_t) c * m] = v;
+            B[i + (int64_t) c * m] = v;
+        }
+        __syncthreads();
+        wg_apply_q<T, PB>(mu, kt, w.j, mu, w.t_j, B, m, c0, c1, lds);
+        __syncthreads();
+        HCAMD_SUB_BEGIN(t_aq);
+        wg_apply_q<T, PB>(m, mu, w.us, m, w.t_u, B, m, c0, c1, lds);
+        HCAMD_SUB_END(t_aq, w.hdr, 4);
+    } else if (side == 0) {
+        T *B = d.cdata;  // ld m
+        // the core's left factor Z / sigma in rows [0, mu) (J's rows are in Q_U's basis order)
+        for (int64_t e = threadIdx.x; e < (int64_t) m * (c1 - c0); e += NT) {
+            const int i = (int) (e % m), c = c0 + (int) (e / m);
+            T v = T(0);
+            if (i < mu && c < kt) {
+                const int j = w.perm[c];
+                const T sg = w.sig[j];
+                v = (sg > T(0)) ? w.j[i + (int64_t) j * mu] / sg : T(0);
+            }
+            B[i + (int64_t) c * m] = v;
         }
         __syncthreads();
         HCAMD_SUB_BEGIN(t_aq);
