@@ -467,10 +467,11 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
     HCAMD_SUB_BEGIN(t_piv);
     if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P, cn);
     HCAMD_SUB_END(t_piv, hdr, 5);
-    for (int e = threadIdx.x; e < rows * PB; e += NT) {
-        const int r = e % rows, c = e / rows;
-        P[e] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
-    }
+    // (column-outer loops over the compile-time panel width: no per-element index division)
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+        for (int r = threadIdx.x; r < rows; r += NT)
+            P[r + c * rows] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
     __syncthreads();
     for (int jj = 0; jj < jb; ++jj) {
         T part = T(0);
@@ -523,30 +524,26 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
                 }
             }
             __syncthreads();
-            for (int e = threadIdx.x; e < rows * PB; e += NT) {
-                const int r = e % rows, k = e / rows;
-                if (k > jj && k < jb && r >= jj) {
-                    const T v = (r == jj) ? T(1) : P[r + jj * rows];
-                    P[e] -= gram[k] * v;
-                }
+            for (int r = jj + threadIdx.x; r < rows; r += NT) {
+                const T v = (r == jj) ? T(1) : P[r + jj * rows];
+#pragma unroll
+                for (int k = 0; k < PB; ++k)
+                    if (k > jj && k < jb) P[r + k * rows] -= gram[k] * v;
             }
             __syncthreads();
         }
     }
     HCAMD_SUB_END(t_pan, hdr, 0);
     // write the factored panel back (R on/above the diagonal, reflectors below)
-    for (int e = threadIdx.x; e < rows * jb; e += NT) {
-        const int r = e % rows, c = e / rows;
-        A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[e];
-    }
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+        if (c < jb)
+            for (int r = threadIdx.x; r < rows; r += NT) A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[r + c * rows];
     __syncthreads();
-    for (int e = threadIdx.x; e < rows * PB; e += NT) {
-        const int r = e % rows, c = e / rows;
-        if (c < jb) {
-            if (r == c) P[e] = T(1);
-            else if (r < c) P[e] = T(0);
-        }
-    }
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+        if (c < jb)
+            for (int r = threadIdx.x; r <= c && r < rows; r += NT) P[r + c * rows] = (r == c) ? T(1) : T(0);
     __syncthreads();
     HCAMD_SUB_BEGIN(t_bt);
     build_t<T, PB>(P, rows, tau + j0, jb, Tm, gram);
@@ -591,15 +588,16 @@ __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *ts
         const int j0 = p * PB;
         const int jb = (nref - j0) < PB ? (nref - j0) : PB;
         const int rows = m - j0;
-        for (int e = threadIdx.x; e < rows * PB; e += NT) {
-            const int r = e % rows, c = e / rows;
-            T v = T(0);
-            if (c < jb) {
-                if (r == c) v = T(1);
-                else if (r > c) v = A[(j0 + r) + (int64_t) (j0 + c) * lda];
+#pragma unroll
+        for (int c = 0; c < PB; ++c)
+            for (int r = threadIdx.x; r < rows; r += NT) {
+                T v = T(0);
+                if (c < jb) {
+                    if (r == c) v = T(1);
+                    else if (r > c) v = A[(j0 + r) + (int64_t) (j0 + c) * lda];
+                }
+                P[r + c * rows] = v;
             }
-            P[e] = v;
-        }
         for (int e = threadIdx.x; e < PB * PB; e += NT) Tm[e] = tsave[(int64_t) p * PB * PB + e];
         __syncthreads();
         apply_wy_dispatch<T, PB>(P, Tm, rows, B + j0, ldb, c0, c1, false, red);
