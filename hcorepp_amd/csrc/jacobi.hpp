@@ -15,12 +15,31 @@
 // Convergence test: |z_p . z_q| <= sqrt(rows) * eps * |z_p| |z_q| (relative,
 // as LAPACK's one-sided Jacobi), pairs involving a column whose norm is below
 // eps * ||J||_F are not rotated (those columns carry rounding-level
-// singular values).
+// singular values).  A sweep ends the iteration when none of its rotations
+// has |sin| above 0.1 sqrt(eps): Jacobi converges quadratically, so the pairs'
+// remaining cosines are then of order ncols * (0.1 sqrt(eps))^2, below the
+// tolerance, and the sweep that would only confirm it is skipped.
 #pragma once
 
 #include "device_common.hpp"
 
 namespace hcamd {
+
+// Bits of the rotation summary the Jacobi rounds and tiles return: some pair was rotated (the
+// tile must be written back), and some rotation was above the small-angle bound (the sweep has
+// not converged).
+enum : int { JROT_APPLIED = 1, JROT_LARGE = 2 };
+
+template <typename T>
+struct JacobiSmallAngle;
+template <>
+struct JacobiSmallAngle<double> {
+    static constexpr double value = 1.4901161193847656e-09;  // 0.1 * sqrt(2^-52)
+};
+template <>
+struct JacobiSmallAngle<float> {
+    static constexpr float value = 3.4526698e-05f;  // 0.1 * sqrt(2^-23)
+};
 
 // Lane that holds value index j after wave_multi_sum<NV>: index bits map to lane bits 5..(6-log2 NV).
 template <int NV>
@@ -120,7 +139,7 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], 
                 vt[r][p] = ck * va - sk * vb;
                 vt[r][q] = sk * va + ck * vb;
             }
-            any = 1;
+            any |= JROT_APPLIED | (fabs(sk) > JacobiSmallAngle<T>::value ? JROT_LARGE : 0);
         }
     }
     return any;
@@ -158,7 +177,7 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
 #pragma unroll
         for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, RPLV, TW, false>(y, vt, rd, tol, floor2);
     }
-    if (any) {
+    if (any & JROT_APPLIED) {
 #pragma unroll
         for (int c = 0; c < TW; ++c) {
             const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -231,7 +250,7 @@ __device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int
             }
             __syncthreads();
         }
-        if (any && lane_id() == 0) atomicOr(&s_rot, 1);
+        if ((any & JROT_LARGE) && lane_id() == 0) atomicOr(&s_rot, 1);
         __syncthreads();
         const int rot = s_rot;
         __syncthreads();
@@ -304,7 +323,7 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
                 vt[r][p] = ck * va - sk * vb;
                 vt[r][q] = sk * va + ck * vb;
             }
-            any = 1;
+            any |= JROT_APPLIED | (fabs(sk) > JacobiSmallAngle<T>::value ? JROT_LARGE : 0);
         }
     }
     return any;
@@ -350,7 +369,7 @@ __device__ int coop_jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, in
             parity ^= 1;
         }
     }
-    if (any) {
+    if (any & JROT_APPLIED) {
 #pragma unroll
         for (int c = 0; c < TW; ++c) {
             const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -434,7 +453,7 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
                 y[r][p] = ck * a - sk * b;
                 y[r][q] = sk * a + ck * b;
             }
-            any = 1;
+            any |= JROT_APPLIED | (fabs(sk) > JacobiSmallAngle<T>::value ? JROT_LARGE : 0);
         }
     }
     return any;
@@ -508,7 +527,7 @@ __device__ __forceinline__ int coop_jacobi_tile_rec(T *J, int rows, int ncols, i
             parity ^= 1;
         }
     }
-    if (any) {
+    if (any & JROT_APPLIED) {
 #pragma unroll
         for (int c = 0; c < TW; ++c) {
             const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -570,7 +589,7 @@ __device__ int wg_coop_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int6
             }
         }
         // every wave derived the same rotations, so `any` agrees across the workgroup
-        if (!any) break;
+        if (!(any & JROT_LARGE)) break;
     }
     __syncthreads();
     return sweep;

@@ -237,7 +237,7 @@ __global__ void HCAMD_KERNEL k_jacobi_step(RecompDesc<T> d, int32_t *flags, int 
     const T tol = sqrt(T(rows)) * Eps<T>::value;
     const int any = coop_jacobi_tile_rec<T, RPL, MW_TW>(w.j, rows, ncols, rows, w.v1, ncols, bI * B, bJ * B,
                                                          step == 0, tol, floor2, lds);
-    if (any && threadIdx.x == 0) flags[sweep] = 1;
+    if ((any & JROT_LARGE) && threadIdx.x == 0) flags[sweep] = 1;
 }
 
 template <typename T>
