@@ -282,6 +282,7 @@ class RunContext:
         h = C.c_void_p()
         _check(lib().hcorepp_ctx_create(device, arena_bytes, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # batches on this context, released before it
         _live.add(self)
 
     def sync(self):
@@ -303,6 +304,8 @@ class RunContext:
         return float(ms.value)
 
     def _release(self):
+        for child in list(getattr(self, "_children", ())):
+            child._release()
         if getattr(self, "h", None) and not _shutting_down:
             lib().hcorepp_ctx_destroy(self.h)
         self.h = None
@@ -453,6 +456,7 @@ class Batch:
                                       int(precision), C.byref(h)))
         self.h = h
         _live.add(self)
+        ctx._children.add(self)
 
     def run(self):
         _check(lib().hcorepp_batch_run(self.h))

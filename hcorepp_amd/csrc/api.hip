@@ -777,6 +777,9 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     for (int64_t i = 0; i < n; ++i) order[(size_t) i] = (int32_t) i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
     hipStream_t s = ctx->stream;
+    // counters start at zero, so that status reads before the first pass (or of an empty batch) are clean
+    if (launch_reset_counters(s, bt->ctr) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return cleanup(fail(HCOREPP_ERR_HIP, "batch counter reset"));
     if (n > 0) {
         if (hipMemcpyAsync(bt->entries, ent.data(), sizeof(BatchEntry<T>) * n, hipMemcpyHostToDevice, s) !=
                     hipSuccess ||

@@ -250,6 +250,8 @@ hcorepp_status hcorepp_mlp_predict_bf16(hcorepp_ctx *ctx, const uint16_t *model_
                                         float *out);
 /* Bytes of device workspace the batch holds. */
 size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b);
+/* Frees the batch; it synchronises the batch's context stream, so destroy every batch before
+ * the context it was created on. */
 hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b);
 
 /* Diagnostics (no counterpart in the reference): copy the device workspace of the last plain

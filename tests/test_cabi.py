@@ -91,3 +91,25 @@ def test_cpp_dropin_gemm_runs(tmp_path):
     res = subprocess.run([exe, "run"], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "max abs err" in res.stdout
+
+
+def test_argument_checks_fail_before_any_device_work():
+    """Entry points reject bad arguments with HCOREPP_ERR_INVALID and a message, before they
+    touch the GPU (so these run on the CPU)."""
+    import hcorepp_amd as H
+    L = H.lib()
+    t = H._Tile(4, 4, 1, 1, 4, 0, H.COMPRESSED, None)
+    p = H.CompressionParameters(1e-8).c()
+    flops = ctypes.c_uint64(0)
+    st = L.hcorepp_gemm_d(None, 1.0, ctypes.byref(t), b"N", ctypes.byref(t), b"N", 1.0, ctypes.byref(t),
+                          ctypes.byref(p), 0, ctypes.byref(flops), None, None)
+    assert st == -1 and b"context" in L.hcorepp_last_error()
+    assert L.hcorepp_potrf_d(None, ctypes.byref(t)) == -1
+    assert L.hcorepp_trsm_batch_d(None, 1, ctypes.byref(t), ctypes.byref(t)) == -1
+    assert L.hcorepp_syrk_batch_d(None, 1, ctypes.byref(t), ctypes.byref(t)) == -1
+    assert L.hcorepp_transpose_batch_d(None, 1, ctypes.byref(t), ctypes.byref(t)) == -1
+    assert L.hcorepp_compress_d(None, 4, 4, None, 4, ctypes.byref(p), ctypes.byref(t)) == -1
+    assert L.hcorepp_kernel_gemm_d(None, b"N", b"N", 2, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2) == -1
+    out = ctypes.c_void_p()
+    assert L.hcorepp_batch_create(None, 1, ctypes.byref(t), ctypes.byref(t), ctypes.byref(t), 1.0, 1.0,
+                                  ctypes.byref(p), 0, ctypes.byref(out)) == -1
