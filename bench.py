@@ -423,6 +423,16 @@ def main():
     flops = float(np.sum([alg_flops(NB, a, b, c, r) for a, b, c, r in zip(ra, rb, rc, ranks)]))
     traffic_alg = float(np.sum([alg_bytes(NB, a, b, c, r) for a, b, c, r in zip(ra, rb, rc, ranks)]))
     achieved_tflops = flops / (hb["kernel_ms"] * 1e-3) / 1e12
+    # HBM bytes per launch: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and form, calibrated
+    # against a copy of known size (the committed profile gives the commands); bench.py cannot
+    # collect counters itself, so the figure is quoted from there for the same form
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", f"k_batched_ccc{args.updates}_nb{NB}_{best}.md")
+    if os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":")[1])
+                traffic_src = f"rocprofv3 PMC, {os.path.relpath(prof, ROOT)}"
     describe = {
         "fp64_exact": "batched FP64, exact core, persistent kernel",
         "fp64_learned": f"batched FP64, core presized by the BF16 rank MLP (margin {LEARNED_MARGIN}, accepted when "
@@ -478,7 +488,8 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_batched (persistent batched recompression)",
             "kernel_ms": round(hb["kernel_ms"], 3),
             "alg_flops_per_launch": flops,
