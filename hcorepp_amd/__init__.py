@@ -59,7 +59,7 @@ EXPORTED = [
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
     "hcorepp_batch_workspace_bytes", "hcorepp_batch_phase_times", "hcorepp_batch_set_grid", "hcorepp_batch_get_grid",
-    "hcorepp_batch_set_rank_model", "hcorepp_batch_features", "hcorepp_batch_learned_stats",
+    "hcorepp_batch_set_rank_model", "hcorepp_batch_features", "hcorepp_batch_core_widths", "hcorepp_batch_learned_stats",
     "hcorepp_rank_model_features", "hcorepp_rank_model_params", "hcorepp_mlp_workspace_floats",
     "hcorepp_mlp_forward_f32", "hcorepp_mlp_grad_f32", "hcorepp_mlp_adam_f32", "hcorepp_mlp_pack_bf16",
     "hcorepp_mlp_predict_bf16", "hcorepp_potrf_d", "hcorepp_trsm_batch_d", "hcorepp_syrk_batch_d",
@@ -134,6 +134,7 @@ def lib():
     L.hcorepp_batch_set_rank_model.argtypes = [P, P, D, C.c_int]
     L.hcorepp_batch_features.argtypes = [P, P]
     L.hcorepp_batch_learned_stats.argtypes = [P, C.POINTER(I64), P]
+    L.hcorepp_batch_core_widths.argtypes = [P, P]
     L.hcorepp_rank_model_params.restype = C.c_size_t
     L.hcorepp_mlp_workspace_floats.argtypes = [I64]
     L.hcorepp_mlp_workspace_floats.restype = C.c_size_t
@@ -510,6 +511,12 @@ class Batch:
         pred = np.zeros(self.n, dtype=np.int64)
         _check(lib().hcorepp_batch_learned_stats(self.h, C.byref(fb), pred.ctypes.data_as(C.c_void_p)))
         return int(fb.value), pred
+
+    def core_widths(self) -> np.ndarray:
+        """Core width each update of the last pass handed to its SVD."""
+        out = np.zeros(self.n, dtype=np.int64)
+        _check(lib().hcorepp_batch_core_widths(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def phase_times(self) -> np.ndarray:
         """Per-update phase timestamps and sub-phase durations of the last pass (n x 16,

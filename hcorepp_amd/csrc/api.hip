@@ -931,6 +931,15 @@ hcorepp_status hcorepp_batch_learned_stats(hcorepp_batch *b, int64_t *fallbacks,
     return HCOREPP_OK;
 }
 
+hcorepp_status hcorepp_batch_core_widths(hcorepp_batch *b, int64_t *widths) {
+    if (!b || (b->n > 0 && !widths)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    for (int64_t i = 0; i < b->n; ++i)
+        HIP_TRY(hipMemcpy(widths + i, b->ws + b->hdr_off[(size_t) i] + HDR_KT * 8, sizeof(int64_t),
+                          hipMemcpyDeviceToHost));
+    return HCOREPP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // TLR Cholesky tile operations (FP64).
 extern "C++" {

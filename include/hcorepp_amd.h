@@ -232,6 +232,9 @@ hcorepp_status hcorepp_batch_features(hcorepp_batch *b, float *features);
 /* Fallbacks of the last pass (synchronises); predicted (may be NULL, host, n entries) gets each
  * update's predicted rank (0 when no model was set). */
 hcorepp_status hcorepp_batch_learned_stats(hcorepp_batch *b, int64_t *fallbacks, int64_t *predicted);
+/* Width of the core each update of the last pass handed to its SVD (synchronises; host, n entries):
+ * the Jacobi's work grows with its square, so this is what learned or rule sizing saves. */
+hcorepp_status hcorepp_batch_core_widths(hcorepp_batch *b, int64_t *widths);
 int hcorepp_rank_model_features(void);
 size_t hcorepp_rank_model_params(void);
 /* FP32 training kernels on device buffers: X (n x features, row-major), y (n), workspace of

@@ -118,3 +118,43 @@ def test_learned_core_sizing_matches_exact_and_falls_back(ctx):
         assert rel_fro(Cr[i].to_dense(), exact[i]) < 1e-9
     for x in (be, bl, bb, br):
         x.close()
+
+
+def test_learned_sizing_gain_on_benchmark_distribution(ctx):
+    """The committed rank model (hcorepp_amd/rank_mlp.npz) on 1024 updates of the benchmark's
+    distribution (nb=512, ranks log-uniform [8, 192]): same ranks as the exact form, results within
+    1e-8, and narrower cores: the Jacobi's work (sum of cubed core widths) drops.  Pass times of
+    both forms are printed."""
+    import os
+    import time
+    H, Lm, M = _mods()
+    from hcorepp_amd.workloads import batch_ccc_pool
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hcorepp_amd", "rank_mlp.npz")
+    model = Lm.RankPredictor.load(ctx, path)
+    ups = batch_ccc_pool(1024, 512, seed=4242, lo=8, hi=192, decay=0.85)
+    bw, _ = _batch(H, ctx, ups[:64], acc=1e-8, mr=384)  # warm-up (code objects, arena)
+    bw.run()
+    ctx.sync()
+    bw.close()
+    forms = {}
+    for name in ("exact", "learned"):
+        b, Cs = _batch(H, ctx, ups, acc=1e-8, mr=384)
+        if name == "learned":
+            b.set_rank_model(model.bf16(), tol=0.1, margin=40)
+        ctx.sync()
+        t = time.perf_counter()
+        b.run()
+        ctx.sync()
+        dt = time.perf_counter() - t
+        forms[name] = (b, Cs, b.core_widths(), dt)
+    (be, Ce, ke, te), (bl, Cl, kl, tl) = forms["exact"], forms["learned"]
+    print(f"exact {te * 1e3:.1f} ms, learned {tl * 1e3:.1f} ms; sum kt^2 {np.sum(ke ** 2.0):.3e} -> "
+          f"{np.sum(kl ** 2.0):.3e}; fallbacks {bl.learned_stats()[0]}")
+    re, rl = be.sync_ranks_to_tiles(), bl.sync_ranks_to_tiles()
+    assert np.array_equal(re, rl)
+    for i in range(0, len(ups), 16):
+        assert rel_fro(Cl[i].to_dense(), Ce[i].to_dense()) < 1e-8
+    # the Jacobi on a kt x kt core (after the core QR) costs ~ kt^3 per sweep
+    assert np.sum(kl ** 3.0) < 0.97 * np.sum(ke ** 3.0)
+    be.close()
+    bl.close()
