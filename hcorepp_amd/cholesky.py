@@ -88,9 +88,16 @@ class TLRCholesky:
                 progress(k)
         self.ctx.sync()
 
+    # Steps with at most this many trailing updates run them one by one in the plain form (whose
+    # QR and Jacobi phases spread over the whole GPU): a persistent batch of a few large updates
+    # lasts as long as its largest update on one workgroup (~100-150 ms at nb = 512, K = 340), the
+    # plain form ~15 ms per update.
+    PLAIN_STEP_MAX_UPDATES = 7
+
     def factor_batched(self, params: CompressionParameters, precision=PREC_FP64, tol=0.0):
         """Per step one launch per tile operation kind; the trailing updates as one persistent
-        recompression batch (optionally with the pivoted-R tolerance rule)."""
+        recompression batch (optionally with the pivoted-R tolerance rule); the last steps, with
+        a handful of updates, in the plain form (FP64 exact core)."""
         for k in range(self.T):
             rows = list(range(k + 1, self.T))
             self._potrf(k)
@@ -100,7 +107,10 @@ class TLRCholesky:
             self._syrk(k, rows)
             self._transpose(k, rows)
             pairs = [(i, j) for j in rows for i in range(j + 1, self.T)]
-            if pairs:
+            if 0 < len(pairs) <= self.PLAIN_STEP_MAX_UPDATES and precision == PREC_FP64 and tol == 0.0:
+                for i, j in pairs:
+                    HCore.gemm(-1.0, self.L[(i, k)], self.LT[(j, k)], 1.0, self.L[(i, j)], self.ctx, params)
+            elif pairs:
                 b = Batch(self.ctx, [self.L[(i, k)] for i, j in pairs], [self.LT[(j, k)] for i, j in pairs],
                           [self.L[(i, j)] for i, j in pairs], -1.0, 1.0, params, precision)
                 if tol > 0:

@@ -116,9 +116,9 @@ def test_tile_ops_random_match_oracle(ctx, nb):
         assert np.array_equal(tu, vg.T) and np.array_equal(tv, ug.T)
 
 
-def _small(acc):
+def _small(acc, n=512):
     from hcorepp_amd.workloads import TLRMatrix
-    M = TLRMatrix(512, 128, 2, 0.1, acc, 20261205, nugget=1e-2)
+    M = TLRMatrix(n, 128, 2, 0.1, acc, 20261205, nugget=1e-2)
     diag = [M.tile(k, k)[1] for k in range(M.T)]
     off = {(i, k): M.tile(i, k)[1:] for i in range(M.T) for k in range(i)}
     return M, OC_factor(diag, off, acc)
@@ -129,24 +129,30 @@ def OC_factor(diag, off, acc):
     return OC.TLRCholesky(diag, off, acc).factor()
 
 
-@pytest.mark.parametrize("form", ["plain", "batched"])
+@pytest.mark.parametrize("form", ["plain", "batched", "hybrid"])
 @pytest.mark.parametrize("acc", [1e-5, 1e-8])
 def test_tlr_cholesky_matches_oracle(ctx, form, acc):
     """The device factor equals the oracle's tile sequence to rounding (ranks may differ by a
-    tie at the threshold; the factors then differ by O(acc)); backward error O(acc)."""
+    tie at the threshold; the factors then differ by O(acc)); backward error O(acc).  'batched'
+    runs every step's trailing updates as one batch; 'hybrid' (8 x 8 tiles) the default split,
+    batches for the first steps and the plain form for the last ones."""
     H, HC, OC = _mods()
-    M, F_o = _small(acc)
+    print(f"[case] cholesky {form} acc {acc}", flush=True)
+    M, F_o = _small(acc, n=1024 if form == "hybrid" else 512)
     F = HC.TLRCholesky(ctx, M)
     p = H.CompressionParameters(acc)
     if form == "plain":
         F.factor_plain(p)
+    elif form == "batched":
+        F.PLAIN_STEP_MAX_UPDATES = 0
+        F.factor_batched(p)
     else:
         F.factor_batched(p)
     Ld, Lo = F.dense_factor(), F_o.dense_factor()
     rg, ro = F.ranks(), F_o.ranks()
     diff = sum(abs(rg[k] - ro[k]) for k in ro)
     print("ranks gpu", rg, "oracle", ro)
-    assert diff <= 2
+    assert diff <= 2 + len(ro) // 10
     assert rel_fro(Ld, Lo) < (1e-12 if diff == 0 else 10 * acc)
     A = M.dense_block_range(0, M.n)
     be = np.linalg.norm(A - Ld @ Ld.T) / np.linalg.norm(A)
