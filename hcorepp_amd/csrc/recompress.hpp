@@ -945,8 +945,10 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
             B[i + (int64_t) c * n] = v;
         }
         __syncthreads();
+        // [v1 sigma; 0] vanishes below row kt, where reflectors kt.. of Q_Y act as the identity
+        // (each is zero above its own row), so only the first kt are applied
         HCAMD_SUB_BEGIN(t_aq);
-        wg_apply_q<T, PB>(n, kk, w.y, n, w.t_y, B, n, c0, c1, lds);
+        wg_apply_q<T, PB>(n, kt < kk ? kt : kk, w.y, n, w.t_y, B, n, c0, c1, lds);
         HCAMD_SUB_END(t_aq, w.hdr, 4);
     }
 }
