@@ -65,6 +65,8 @@ enum : int {
     HDR_FEAT = 24,   // RANK_FEATURES float32 rank-predictor features (12 words)
     HDR_PRED = 36,   // predicted rank (learned core sizing), 0 when off
     HDR_LEARNED = 37,  // 1: core sized by the prediction, 2: prediction rejected (fallback), 0: off
+    HDR_KQ = 38,     // rows of R_Y the QR of Y computed when it stopped early (0: all of them)
+    HDR_TRAIL2 = 39,  // squared Frobenius norm of the part it left unfactored, as double bits
     HDR_WORDS = 40,
 };
 enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2, STATUS_TOO_BIG = 4 };
@@ -192,6 +194,8 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
         w.hdr[HDR_STATUS] = (K > d.kmax) ? STATUS_TOO_BIG : 0;
         w.hdr[HDR_PRED] = 0;
         w.hdr[HDR_LEARNED] = 0;
+        w.hdr[HDR_KQ] = 0;
+        w.hdr[HDR_TRAIL2] = 0;
     }
     if (K > d.kmax) return;
     const T *cu = d.cdata;
@@ -364,8 +368,8 @@ __host__ __device__ inline int64_t qr_lds_elems(int64_t m, int pb) {
 // by jb column swaps (whole columns, so the R rows above j0 move with them); piv records
 // the permutation.  Uses the panel buffer P as scratch (needs m * PB >= 2 * (ncols - j0) + 2 * PB).
 template <typename T, int PB>
-__device__ void window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, int32_t *piv, T *P,
-                             const T *cn = nullptr) {
+__device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, int32_t *piv, T *P,
+                          const T *cn = nullptr) {
     const int nt = ncols - j0;
     const int l = lane_id();
     T *nrm = P;
@@ -391,6 +395,11 @@ __device__ void window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb
         at[c] = c;
         where[c] = c;
     }
+    __syncthreads();
+    // squared Frobenius norm of the trailing matrix A[j0:m, j0:ncols] (the stop test of ph_qr_y)
+    T tpart = T(0);
+    for (int c = threadIdx.x; c < nt; c += NT) tpart += nrm[c];
+    const T trail2 = wg_sum(tpart, reinterpret_cast<T *>(sel + 2 * PB) + 1);
     __syncthreads();
     for (int c = threadIdx.x; c < nt; c += NT) {
         const T v = nrm[c];
@@ -432,6 +441,7 @@ __device__ void window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb
         }
         __syncthreads();
     }
+    return trail2;
 }
 
 // Whether window pivoting fits the panel buffer of a QR of m x ncols with panel PB.
@@ -605,8 +615,28 @@ __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *ts
     }
 }
 
+// Squared threshold below which the trailing part of the pivoted QR of Y may be left unfactored:
+// the rows of R_Y it would produce are the ones the core truncation drops anyway -- at the
+// rounding level eps ||Y||_F (when the rank threshold is far above it, as in ph_core_size), or at
+// the tolerance rule's tol * threshold.  Negative: factor everything (fixed rank, learned sizing
+// beyond the eps level, thresholds near rounding).
+template <typename T>
+__device__ T qr_y_stop2(const RecompDesc<T> &d, T fro2, int kk) {
+    if (d.fixed_rank > 0 || kk <= 0) return T(-1);
+    const T delta = Eps<T>::value * sqrt(fro2);
+    const T thr_lo = d.truncated_svd ? T(d.accuracy) * sqrt(fro2 / T(kk)) : T(d.accuracy);
+    T stop2 = (thr_lo > T(64) * delta) ? delta * delta : T(-1);
+    if (!d.rank_model && d.model_tol > 0.0) {
+        const T lim = T(d.model_tol) * thr_lo;
+        if (lim * lim > stop2) stop2 = lim * lim;
+    }
+    return stop2;
+}
+
 // Phase: window-pivoted QR of Y (n x mu); piv is the identity when the panel buffer is too
-// small for pivoting (very wide Y).
+// small for pivoting (very wide Y).  With pivoting the factorisation stops at the first panel
+// whose trailing matrix is below qr_y_stop2 (rows of R_Y computed: HDR_KQ, the unfactored norm:
+// HDR_TRAIL2); its Q_Y reflectors beyond that point would only act on rows the core drops.
 template <typename T, int PB>
 __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
@@ -615,8 +645,34 @@ __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
     if (!pivot) {
         for (int c = threadIdx.x; c < z.mu; c += NT) w.piv[c] = c;
         __syncthreads();
+        wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds, nullptr, w.hdr);
+        return;
     }
-    wg_geqrf<T, PB>(d.n, z.mu, w.y, d.n, w.tau_y, w.t_y, lds, pivot ? w.piv : nullptr, w.hdr);
+    const int m = d.n, ncols = z.mu;
+    const int nsteps = m < ncols ? m : ncols;
+    T *P = lds;
+    T *Tm = P + (int64_t) m * PB;
+    T *red = Tm + 2 * PB * PB;
+    for (int c = threadIdx.x; c < ncols; c += NT) w.piv[c] = c;
+    __syncthreads();
+    T stop2 = T(-1);
+    for (int j0 = 0; j0 < nsteps; j0 += PB) {
+        const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
+        const T trail2 = window_pivot<T, PB>(m, ncols, w.y, m, j0, jb, w.piv, P);
+        if (j0 == 0) {
+            stop2 = qr_y_stop2<T>(d, trail2, z.kk);
+        } else if (trail2 <= stop2) {
+            if (threadIdx.x == 0) {
+                w.hdr[HDR_KQ] = j0;
+                w.hdr[HDR_TRAIL2] = __double_as_longlong((double) trail2);
+            }
+            __syncthreads();
+            return;
+        }
+        wg_geqrf_panel<T, PB>(m, ncols, w.y, m, w.tau_y, w.t_y, lds, nullptr, w.hdr, j0);
+        if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, m - j0, w.y + j0, m, j0 + jb, ncols, true, red);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -647,8 +703,11 @@ __device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int mu = z.mu, kk = z.kk, n = d.n, l = lane_id();
+    // rows of R_Y the QR of Y computed (all kk unless it stopped early) and the norm it left
+    const int kq = (w.hdr[HDR_KQ] > 0 && w.hdr[HDR_KQ] < kk) ? (int) w.hdr[HDR_KQ] : kk;
+    const T trail2 = kq < kk ? (T) __longlong_as_double(w.hdr[HDR_TRAIL2]) : T(0);
     T *rn = lds;  // suffix sums, then the threshold bound at rn[kk]
-    for (int j = wave_id(); j < kk; j += NWAVES) {
+    for (int j = wave_id(); j < kq; j += NWAVES) {
         T sum = T(0);
         for (int c = j + l; c < mu; c += WAVE) {
             const T v = w.y[j + (int64_t) c * n];
@@ -659,8 +718,9 @@ __device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        T acc = T(0);
-        for (int j = kk - 1; j >= 0; --j) {
+        for (int j = kq; j < kk; ++j) rn[j] = trail2;  // rows never formed: bounded by what was left
+        T acc = trail2;
+        for (int j = kq - 1; j >= 0; --j) {
             acc += rn[j];
             rn[j] = acc;  // ||R_Y[j:, :]||_F^2
         }
@@ -681,6 +741,7 @@ __device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
             const int fr = d.fixed_rank < kk ? d.fixed_rank : kk;
             if (kt < fr) kt = fr;
         }
+        if (kt > kq) kt = kq;  // only kq rows of R_Y exist (the rest were left below the stop test)
         if (kt < 1) kt = kk < 1 ? 0 : 1;
         w.hdr[HDR_KT] = kt;
         w.hdr[HDR_CQR] = 0;
