@@ -99,8 +99,28 @@ __global__ void HCAMD_KERNEL k_qr_panel(RecompDesc<T> d, int which, int j0, T *v
         for (int c = threadIdx.x; c < q.ncols; c += NT) w.piv[c] = c;
         __syncthreads();
     }
-    wg_geqrf_panel<T, PB>(q.m, q.ncols, q.A, q.lda, q.tau, q.tsave, lds, q.pivot ? w.piv : nullptr, w.hdr, j0,
-                          q.pivot ? cn : nullptr);
+    if (which == 1 && q.pivot) {
+        // the QR of Y stops at the first panel whose trailing matrix is below the core-truncation
+        // level (as ph_qr_y); the launches after that return at once
+        if (w.hdr[HDR_KQ] > 0) return;
+        const int nsteps = q.m < q.ncols ? q.m : q.ncols;
+        const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
+        const T trail2 = window_pivot<T, PB>(q.m, q.ncols, q.A, q.lda, j0, jb, w.piv, lds, cn);
+        if (j0 == 0) {
+            if (threadIdx.x == 0) w.hdr[HDR_STOP2] = __double_as_longlong((double) qr_y_stop2<T>(d, trail2, z.kk));
+        } else if (trail2 <= (T) __longlong_as_double(w.hdr[HDR_STOP2])) {
+            if (threadIdx.x == 0) {
+                w.hdr[HDR_KQ] = j0;
+                w.hdr[HDR_TRAIL2] = __double_as_longlong((double) trail2);
+            }
+            return;
+        }
+        __syncthreads();
+        wg_geqrf_panel<T, PB>(q.m, q.ncols, q.A, q.lda, q.tau, q.tsave, lds, nullptr, w.hdr, j0);
+    } else {
+        wg_geqrf_panel<T, PB>(q.m, q.ncols, q.A, q.lda, q.tau, q.tsave, lds, q.pivot ? w.piv : nullptr, w.hdr, j0,
+                              q.pivot ? cn : nullptr);
+    }
     const int rows = q.m - j0;
     for (int e = threadIdx.x; e < rows * PB; e += NT) vpan[e] = lds[e];
 }
@@ -112,6 +132,7 @@ __global__ void HCAMD_KERNEL k_qr_update(RecompDesc<T> d, int which, int j0, int
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const QrOperand<T> q = qr_operand<T, PB>(d, w, z, which);
+    if (which == 1 && w.hdr[HDR_KQ] > 0) return;  // the QR of Y stopped early
     const int nsteps = q.m < q.ncols ? q.m : q.ncols;
     const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
     const int c0 = j0 + jb + (int) blockIdx.x * cols_per_wg;

@@ -67,7 +67,8 @@ enum : int {
     HDR_LEARNED = 37,  // 1: core sized by the prediction, 2: prediction rejected (fallback), 0: off
     HDR_KQ = 38,     // rows of R_Y the QR of Y computed when it stopped early (0: all of them)
     HDR_TRAIL2 = 39,  // squared Frobenius norm of the part it left unfactored, as double bits
-    HDR_WORDS = 40,
+    HDR_STOP2 = 40,  // the stop test's threshold (multi-workgroup QR of Y), as double bits
+    HDR_WORDS = 48,
 };
 enum : int { STATUS_NOCONV = 1, STATUS_NAN = 2, STATUS_TOO_BIG = 4 };
 
