@@ -97,3 +97,33 @@ def test_error_paths(ctx):
     # compression beyond the core-SVD limit
     with pytest.raises(H.HCoreError, match="UNSUPPORTED"):
         H.CompressedTile.compress(ctx, rng.standard_normal((800, 800)), H.CompressionParameters(1e-8))
+
+
+@pytest.mark.parametrize("kw", [dict(fixed_rank=9), dict(truncated_svd=True, accuracy=1e-6)])
+def test_batched_rank_rules_match_plain(ctx, kw):
+    """Fixed rank and the relative (truncated_svd) rule in the batched form: same ranks and
+    results as the plain form."""
+    H = _H()
+    rng = np.random.default_rng(31)
+    acc = kw.pop("accuracy", 1e-8)
+    p = H.CompressionParameters(acc, **kw)
+    As, Bs, Cb, Cp = [], [], [], []
+    for m, ra, rb, rc in [(96, 12, 10, 14), (128, 20, 18, 25), (256, 30, 20, 40)]:
+        au, av = _lowrank(rng, m, m, ra, decay=0.9)
+        bu, bv = _lowrank(rng, m, m, rb, decay=0.9)
+        cu, cv = _lowrank(rng, m, m, rc, decay=0.9)
+        As.append(H.CompressedTile(au, av))
+        Bs.append(H.CompressedTile(bu, bv))
+        Cb.append(H.CompressedTile(cu, cv, max_rank=m // 2))
+        Cp.append(H.CompressedTile(cu, cv, max_rank=m // 2))
+    b = H.Batch(ctx, As, Bs, Cb, 1.0, 1.0, p)
+    b.run()
+    rk = b.sync_ranks_to_tiles()
+    assert b.status()[0] == 0
+    for i in range(len(As)):
+        H.HCore.gemm(1.0, As[i], Bs[i], 1.0, Cp[i], ctx, p)
+        assert rk[i] == Cp[i].rank
+        if "fixed_rank" in kw:
+            assert rk[i] == kw["fixed_rank"]
+        assert rel_fro(Cb[i].to_dense(), Cp[i].to_dense()) < 1e-11
+    b.close()
