@@ -289,6 +289,11 @@ def main():
 
     batch = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, params, H.PREC_FP64)
     log(f"[bench] batch workspace {batch.workspace_bytes() / 2**30:.2f} GiB")
+    # launch shape: the autotune table's entry for this class (tallest QR NB rows, mean K, updates per CU)
+    tuned, _, _ = H.autotune_lookup(NB, float(np.mean(ra + rc)), args.updates / ctx.compute_units())
+    shape = {"panel": batch.panel(), "workgroups": batch.grid(),
+             "source": "autotune table (hcorepp_amd/autotune.json)" if tuned else "built-in rule"}
+    log(f"[bench] launch shape: panel {shape['panel']}, {shape['workgroups']} workgroups ({shape['source']})")
     if use_graph:
         batch.capture()
     forms = {}
@@ -472,6 +477,7 @@ def main():
             "form_name": best,
             "global_batch": args.updates,
             "parallelism": "single GPU",
+            "launch_shape": shape,
         },
         "plain_fp64_updates_per_s": round(plain_full_est, 2),
         "speedup_vs_plain_fp64": round(hb["ups"] / plain_full_est, 2),

@@ -53,12 +53,13 @@ class _CParams(C.Structure):
 # Every symbol include/hcorepp_amd.h declares.
 EXPORTED = [
     "hcorepp_ctx_create", "hcorepp_ctx_destroy", "hcorepp_ctx_sync", "hcorepp_ctx_stream",
-    "hcorepp_ctx_arena_bytes", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
+    "hcorepp_ctx_arena_bytes", "hcorepp_ctx_compute_units", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
     "hcorepp_memcpy_async", "hcorepp_memset", "hcorepp_kernel_gemm_d", "hcorepp_compress_d", "hcorepp_compress_s",
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
     "hcorepp_batch_workspace_bytes", "hcorepp_batch_phase_times", "hcorepp_batch_set_grid", "hcorepp_batch_get_grid",
+    "hcorepp_batch_set_panel", "hcorepp_batch_get_panel", "hcorepp_autotune_lookup",
     "hcorepp_batch_set_rank_model", "hcorepp_batch_features", "hcorepp_batch_core_widths", "hcorepp_batch_learned_stats",
     "hcorepp_rank_model_features", "hcorepp_rank_model_params", "hcorepp_mlp_workspace_floats",
     "hcorepp_mlp_forward_f32", "hcorepp_mlp_grad_f32", "hcorepp_mlp_adam_f32", "hcorepp_mlp_pack_bf16",
@@ -101,6 +102,8 @@ def lib():
     L.hcorepp_ctx_stream.restype = P
     L.hcorepp_ctx_arena_bytes.argtypes = [P]
     L.hcorepp_ctx_arena_bytes.restype = C.c_size_t
+    L.hcorepp_ctx_compute_units.argtypes = [P]
+    L.hcorepp_ctx_compute_units.restype = C.c_int
     L.hcorepp_memcpy.argtypes = [P, P, P, C.c_size_t]
     L.hcorepp_memset.argtypes = [P, P, C.c_int, C.c_size_t]
     L.hcorepp_memcpy_async.argtypes = [P, P, P, C.c_size_t]
@@ -149,6 +152,11 @@ def lib():
     L.hcorepp_transpose_batch_d.argtypes = [P, I64, C.POINTER(_Tile), C.POINTER(_Tile)]
     L.hcorepp_batch_get_grid.argtypes = [P]
     L.hcorepp_batch_get_grid.restype = C.c_int
+    L.hcorepp_batch_set_panel.argtypes = [P, C.c_int]
+    L.hcorepp_batch_get_panel.argtypes = [P]
+    L.hcorepp_batch_get_panel.restype = C.c_int
+    L.hcorepp_autotune_lookup.argtypes = [I64, D, D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.hcorepp_autotune_lookup.restype = C.c_int
     L.hcorepp_batch_workspace_bytes.argtypes = [P]
     L.hcorepp_batch_workspace_bytes.restype = C.c_size_t
     L.hcorepp_debug_workspace.argtypes = [P, P, C.c_size_t, P]
@@ -162,6 +170,17 @@ def lib():
 def _check(st: int):
     if st != 0:
         raise HCoreError(st, lib().hcorepp_last_error().decode())
+
+
+def autotune_lookup(rows: int, mean_k: float, updates_per_cu: float):
+    """The compiled-in autotune table's choice for a batch whose tallest QR has `rows` rows,
+    whose mean stacked rank (ra + rc) is mean_k and which holds updates_per_cu updates per
+    compute unit: (from_table, panel width, workgroups per CU).  Without a table entry for the
+    class, the built-in rule's choice and from_table False."""
+    pb, wg = C.c_int(0), C.c_int(0)
+    hit = lib().hcorepp_autotune_lookup(int(rows), float(mean_k), float(updates_per_cu), C.byref(pb),
+                                        C.byref(wg))
+    return bool(hit), pb.value, wg.value
 
 
 class DeviceBuffer:
@@ -295,6 +314,9 @@ class RunContext:
 
     def arena_bytes(self) -> int:
         return lib().hcorepp_ctx_arena_bytes(self.h)
+
+    def compute_units(self) -> int:
+        return lib().hcorepp_ctx_compute_units(self.h)
 
     def record(self, slot: int):
         _check(lib().hcorepp_ctx_event_record(self.h, slot))
@@ -491,6 +513,14 @@ class Batch:
 
     def grid(self) -> int:
         return lib().hcorepp_batch_get_grid(self.h)
+
+    def set_panel(self, panel: int):
+        """Householder panel width of the batch's QRs: 4, 8 or 16 within the limits for its
+        tallest QR (drops a captured graph)."""
+        _check(lib().hcorepp_batch_set_panel(self.h, int(panel)))
+
+    def panel(self) -> int:
+        return lib().hcorepp_batch_get_panel(self.h)
 
     def set_rank_model(self, model_bf16, tol: float = 0.1, margin: int = 40):
         """Learned core sizing with a BF16 parameter buffer (DeviceBuffer).  model_bf16=None with

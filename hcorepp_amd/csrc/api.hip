@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "cholesky.hpp"
+#include "autotune_table.hpp"
 
 using namespace hcamd;
 
@@ -443,6 +444,7 @@ struct hcorepp_batch {
     int precision = 0;
     int64_t n = 0;
     int pb = 16;
+    int64_t max_rows = 1;  // tallest stacked-factor QR of the batch
     int grid = 1;
     void *entries = nullptr;  // BatchEntry<double>[n]
     int32_t *order = nullptr;
@@ -455,6 +457,33 @@ struct hcorepp_batch {
     std::vector<int64_t> hdr_off;  // byte offset of each update's workspace header in ws
     std::vector<char> ent_host;    // host copy of the entries (re-uploaded when the rank model changes)
 };
+
+// Autotune table row for a batch class (tallest QR, mean stacked rank, updates per CU): the
+// first row whose bounds hold.  False when the table has none (or is empty).
+static bool tune_lookup(int64_t rows, double mean_k, double per_cu, int *pb, int *wg_per_cu) {
+    for (int i = 0; i < kTuneRows; ++i) {
+        const TuneRow &r = kTuneTable[i];
+        if (rows <= r.max_rows && (r.max_mean_k < 0.0 || mean_k <= r.max_mean_k) &&
+            (r.max_per_cu < 0.0 || per_cu <= r.max_per_cu)) {
+            *pb = r.panel;
+            *wg_per_cu = r.workgroups_per_cu;
+            return true;
+        }
+    }
+    return false;
+}
+
+// A captured graph holds the launch's grid and panel width: changing either drops it.
+static void drop_graph(hcorepp_batch *b) {
+    if (b->graph_exec) {
+        hipGraphExecDestroy(b->graph_exec);
+        b->graph_exec = nullptr;
+    }
+    if (b->graph) {
+        hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+}
 
 extern "C" {
 
@@ -536,6 +565,8 @@ hcorepp_status hcorepp_ctx_event_elapsed(hcorepp_ctx *ctx, int slot0, int slot1,
 void *hcorepp_ctx_stream(hcorepp_ctx *ctx) { return ctx ? (void *) ctx->stream : nullptr; }
 
 size_t hcorepp_ctx_arena_bytes(hcorepp_ctx *ctx) { return ctx ? ctx->arena_cap : 0; }
+
+int hcorepp_ctx_compute_units(hcorepp_ctx *ctx) { return ctx ? ctx->num_cus : 0; }
 
 hcorepp_status hcorepp_malloc(size_t bytes, void **out) {
     if (!out) return fail(HCOREPP_ERR_INVALID, "out is null");
@@ -652,7 +683,7 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     }
     // Plan one update: fills `e` (when non-null) with pointers relative to `base`
     // and returns the bytes of workspace the update needs.
-    auto plan = [&](int64_t i, char *base, BatchEntry<T> *e, double *cost, int64_t *rows) -> size_t {
+    auto plan = [&](int64_t i, char *base, BatchEntry<T> *e, double *cost, int64_t *rows, int64_t *kst) -> size_t {
         const hcorepp_tile *a = &A[i], *b = &B[i], *c = &C[i];
         Holder<T> ah[2], bh[2];
         char aop[2], bop[2];
@@ -724,6 +755,7 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
             *cost = kk * kk * (double) std::max(m, nn) + 20.0 * kk * kk * kk;
         }
         if (rows) *rows = std::max(m, nn);
+        if (kst) *kst = arank + c->rank;
         if (e && (ad.rows != m || bd.cols != nn || bd.rows != arank)) return 0;
         return off + aligned(sizeof(T) * L.total);
     };
@@ -731,13 +763,27 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     std::vector<double> cost((size_t) n);
     size_t ws_total = 0;
     int64_t max_rows = 1;
+    double sum_k = 0.0;
     for (int64_t i = 0; i < n; ++i) {
-        int64_t rows = 0;
+        int64_t rows = 0, kst = 0;
         ws_off[(size_t) i] = ws_total;
-        ws_total += plan(i, nullptr, nullptr, &cost[(size_t) i], &rows);
+        ws_total += plan(i, nullptr, nullptr, &cost[(size_t) i], &rows, &kst);
         max_rows = std::max(max_rows, rows);
+        sum_k += (double) kst;
     }
-    const int pb = pick_pb(max_rows, kLdsBytes / (int64_t) sizeof(T));
+    const double mean_k = n > 0 ? sum_k / (double) n : 0.0;
+    const int64_t lds_elems = kLdsBytes / (int64_t) sizeof(T);
+    // Panel width and residency from the autotune table for this (rows, K, size) class; the
+    // built-in rule when the table has no entry or its panel does not fit these tiles.
+    int pb = 0, wg_per_cu = 2;  // two 78 KB-LDS, 256-VGPR workgroups fill a CU
+    int tuned_pb = 0, tuned_wg = 0;
+    const double per_cu = (double) n / (double) std::max(ctx->num_cus, 1);
+    if (tune_lookup(max_rows, mean_k, per_cu, &tuned_pb, &tuned_wg) && pb_fits(max_rows, tuned_pb, lds_elems)) {
+        pb = tuned_pb;
+        wg_per_cu = tuned_wg;
+    } else {
+        pb = pick_pb(max_rows, lds_elems);
+    }
     if (pb == 0) return fail(HCOREPP_ERR_UNSUPPORTED, "tile too tall for the LDS panel");
 
     hcorepp_batch *bt = new (std::nothrow) hcorepp_batch();
@@ -746,9 +792,9 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     bt->precision = precision;
     bt->n = n;
     bt->pb = pb;
+    bt->max_rows = max_rows;
     bt->ws_bytes = ws_total;
-    const int occupancy = 2;  // two 80 KB-LDS workgroups per CU
-    bt->grid = (int) std::max<int64_t>(1, std::min<int64_t>(n, (int64_t) std::max(ctx->num_cus, 1) * occupancy));
+    bt->grid = (int) std::max<int64_t>(1, std::min<int64_t>(n, (int64_t) std::max(ctx->num_cus, 1) * wg_per_cu));
     auto cleanup = [&](hcorepp_status s2) {
         hcorepp_batch_destroy(bt);
         return s2;
@@ -765,7 +811,7 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     for (int64_t i = 0; i < n; ++i) {
         BatchEntry<T> &e = ent[(size_t) i];
         memset(&e, 0, sizeof(e));
-        if (plan(i, bt->ws + ws_off[(size_t) i], &e, nullptr, nullptr) == 0)
+        if (plan(i, bt->ws + ws_off[(size_t) i], &e, nullptr, nullptr, nullptr) == 0)
             return cleanup(fail(HCOREPP_ERR_INVALID, "operand shapes do not match C (update %lld)", (long long) i));
         e.rd.crank = bt->ranks + i;
         ranks[(size_t) i] = C[i].rank;
@@ -805,18 +851,30 @@ hcorepp_status hcorepp_batch_run(hcorepp_batch *b) {
 hcorepp_status hcorepp_batch_set_grid(hcorepp_batch *b, int workgroups) {
     if (!b || workgroups < 1) return fail(HCOREPP_ERR_INVALID, "bad arguments");
     b->grid = (int) std::min<int64_t>(workgroups, std::max<int64_t>(b->n, 1));
-    if (b->graph_exec) {
-        hipGraphExecDestroy(b->graph_exec);
-        b->graph_exec = nullptr;
-    }
-    if (b->graph) {
-        hipGraphDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    drop_graph(b);
     return HCOREPP_OK;
 }
 
 int hcorepp_batch_get_grid(hcorepp_batch *b) { return b ? b->grid : 0; }
+
+hcorepp_status hcorepp_batch_set_panel(hcorepp_batch *b, int pb) {
+    if (!b) return fail(HCOREPP_ERR_INVALID, "null batch");
+    if (!pb_fits(b->max_rows, pb, kLdsBytes / (int64_t) sizeof(double)))
+        return fail(HCOREPP_ERR_INVALID, "panel width %d does not fit a QR of %lld rows", pb, (long long) b->max_rows);
+    b->pb = pb;
+    drop_graph(b);
+    return HCOREPP_OK;
+}
+
+int hcorepp_batch_get_panel(hcorepp_batch *b) { return b ? b->pb : 0; }
+
+int hcorepp_autotune_lookup(int64_t rows, double mean_k, double updates_per_cu, int *pb, int *workgroups_per_cu) {
+    int p = 0, w = 0;
+    const int hit = tune_lookup(rows, mean_k, updates_per_cu, &p, &w) ? 1 : 0;
+    if (pb) *pb = hit ? p : pick_pb(rows, kLdsBytes / (int64_t) sizeof(double));
+    if (workgroups_per_cu) *workgroups_per_cu = hit ? w : 2;
+    return hit;
+}
 
 hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times) {
     if (!b || (!times && b->n > 0)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
@@ -834,14 +892,7 @@ hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times) {
 hcorepp_status hcorepp_batch_capture_graph(hcorepp_batch *b) {
     if (!b) return fail(HCOREPP_ERR_INVALID, "null batch");
     hipStream_t s = b->ctx->stream;
-    if (b->graph_exec) {
-        hipGraphExecDestroy(b->graph_exec);
-        b->graph_exec = nullptr;
-    }
-    if (b->graph) {
-        hipGraphDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hcorepp_status st = hcorepp_batch_run(b);
     hipGraph_t g = nullptr;

@@ -482,11 +482,17 @@ hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A,
 // Householder panel width for a QR of `rows` rows: the widest panel whose register tile in
 // apply_wy (rows/256 row chunks x PB per lane) stays within 32 values and whose LDS fits.
 int pick_pb(int64_t rows, int64_t lds_elems) {
-    if (rows > 2048) return 0;
-    const int pb = rows <= 512 ? 16 : rows <= 1024 ? 8 : 4;
-    for (int p = pb; p >= 4; p /= 2)
-        if (qr_lds_elems(rows, p) <= lds_elems) return p;
+    for (int p = 16; p >= 4; p /= 2)
+        if (pb_fits(rows, p, lds_elems)) return p;
     return 0;
+}
+
+// Whether a panel of pb columns is valid for a QR of `rows` rows: an instantiated width, at most
+// 32 panel values per lane in apply_wy, and the panel workspace within lds_elems.
+bool pb_fits(int64_t rows, int pb, int64_t lds_elems) {
+    if (pb != 4 && pb != 8 && pb != 16) return false;
+    if (rows > 2048 * 4 / pb) return false;
+    return qr_lds_elems(rows, pb) <= lds_elems;
 }
 
 template <typename T, int PB>

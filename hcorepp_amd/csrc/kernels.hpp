@@ -48,6 +48,7 @@ struct BatchCounters {
 };
 
 int pick_pb(int64_t rows, int64_t lds_elems);
+bool pb_fits(int64_t rows, int pb, int64_t lds_elems);
 
 template <typename T>
 hipError_t launch_gemm(hipStream_t s, int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B,

@@ -93,6 +93,8 @@ hcorepp_status hcorepp_ctx_sync(hcorepp_ctx *ctx);
 void *hcorepp_ctx_stream(hcorepp_ctx *ctx);
 /* Bytes currently reserved by the context arena. */
 size_t hcorepp_ctx_arena_bytes(hcorepp_ctx *ctx);
+/* Compute units of the context's device (no counterpart in the reference). */
+int hcorepp_ctx_compute_units(hcorepp_ctx *ctx);
 
 /* Device memory helpers (reference include/hcorepp/kernels/memory.hpp:46-121:
  * AllocateArray, DestroyArray, Memcpy (cudaMemcpyDefault semantics), Memset). */
@@ -188,10 +190,26 @@ hcorepp_status hcorepp_batch_set_ranks(hcorepp_batch *b, const int64_t *ranks);
 /* Worst status over the last pass (synchronises): 0, or HCOREPP_ERR_NUMERIC. Also max Jacobi sweeps. */
 hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps);
 /* Launch shape of the persistent kernel (no counterpart in the reference): number of
- * workgroups pulling updates from the queue (default 2 per CU, at most n).  Setting it drops a
- * captured graph; capture again afterwards. */
+ * workgroups pulling updates from the queue (default: the autotune table's workgroups per CU
+ * for the batch's class times the CU count, at most n).  Setting it drops a captured graph;
+ * capture again afterwards. */
 hcorepp_status hcorepp_batch_set_grid(hcorepp_batch *b, int workgroups);
 int hcorepp_batch_get_grid(hcorepp_batch *b);
+/* Householder panel width of the batch's QRs (no counterpart in the reference; the
+ * reference's QR is LAPACK's, whose block size is LAPACK's own choice).  Default: the
+ * autotune table's width for the batch's class (tallest QR, mean stacked rank K = ra + rc,
+ * updates per CU).
+ * 4, 8 or 16, within the register/LDS limits for the tallest QR (<= 512 rows for 16, <= 1024
+ * for 8, <= 2048 for 4); anything else is HCOREPP_ERR_INVALID.  Drops a captured graph. */
+hcorepp_status hcorepp_batch_set_panel(hcorepp_batch *b, int panel);
+int hcorepp_batch_get_panel(hcorepp_batch *b);
+/* The compiled-in autotune table's choice for a batch whose tallest QR has `rows` rows, whose
+ * mean stacked rank is mean_k and which holds updates_per_cu updates per compute unit (no
+ * counterpart in the reference).  Returns 1 and the table's panel width and workgroups per CU
+ * when the table covers the class, else 0 and the built-in rule's (widest fitting panel, 2
+ * workgroups per CU). */
+int hcorepp_autotune_lookup(int64_t rows, double mean_k, double updates_per_cu, int *panel,
+                            int *workgroups_per_cu);
 /* Diagnostics (no counterpart in the reference): per-update phase timestamps of the last pass,
  * 16 int64 per update in ticks of the 100 MHz device wall clock: 8 timestamps (start, after the
  * factor products, after QR(U_s), after Y, after QR(Y) and the core set-up, after the core SVD,
