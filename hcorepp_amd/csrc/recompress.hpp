@@ -814,16 +814,20 @@ __device__ HCAMD_PHASE_ATTR int coop_jacobi_nl(T *J, int rows, int ncols, T *V, 
 // and V columns; four tiles in flight per workgroup) with shapes that fit 256 VGPRs without
 // spilling; for 193-256 rows the four waves share 16-column tiles instead (rows split across
 // waves, one barrier per rotation round), which measured faster there because it needs half the
-// passes over J and V of the widest per-wave tile that fits.
+// passes over J and V of the widest per-wave tile that fits.  `exact_core` (a core sized at the
+// rounding level only) also takes the shared tiles for 129-192 rows: 7 % faster on the nb=512
+// benchmark batch, while cores cut by a rank tolerance ran 8 % slower with them.
 template <typename T>
-__device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf) {
+__device__ int run_block_jacobi(T *J, int rows, int ncols, T *V, T *gbuf, bool exact_core) {
     if (rows <= 64) return block_jacobi_nl<T, 1, 1, 16>(J, rows, ncols, V, gbuf);
     if (rows <= 128)
         return ncols <= 64 ? block_jacobi_nl<T, 2, 1, 8>(J, rows, ncols, V, gbuf)
                            : block_jacobi_nl<T, 2, 2, 8>(J, rows, ncols, V, gbuf);
-    if (rows <= 192)
+    if (rows <= 192) {
+        if (exact_core) return coop_jacobi_nl<T, 1, 1, 16>(J, rows, ncols, V, gbuf);
         return ncols <= 128 ? block_jacobi_nl<T, 3, 2, 8>(J, rows, ncols, V, gbuf)
                             : block_jacobi_nl<T, 3, 3, 4>(J, rows, ncols, V, gbuf);
+    }
     if (rows <= 256) return coop_jacobi_nl<T, 1, 1, 16>(J, rows, ncols, V, gbuf);
     if (rows <= 320)
         return ncols <= 192 ? block_jacobi_nl<T, 5, 3, 4>(J, rows, ncols, V, gbuf)
@@ -935,8 +939,9 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
         if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
     } else {
         HCAMD_SUB_BEGIN(t_jac);
-        if (w.hdr[HDR_CQR]) sweeps = run_block_jacobi<T>(w.j2, z.kt, z.kt, w.v1, lds);
-        else sweeps = run_block_jacobi<T>(w.j, z.mu, z.kt, w.v1, lds);
+        const bool exact_core = !d.rank_model && !(d.model_tol > 0.0);
+        if (w.hdr[HDR_CQR]) sweeps = run_block_jacobi<T>(w.j2, z.kt, z.kt, w.v1, lds, exact_core);
+        else sweeps = run_block_jacobi<T>(w.j, z.mu, z.kt, w.v1, lds, exact_core);
         HCAMD_SUB_END(t_jac, w.hdr, 3);
 #ifdef HCAMD_PHASE_TIMING
         if (threadIdx.x == 0) w.hdr[HDR_SUB + 7] = sweeps + 1000 * z.kt;  // timing build: sweeps and core width
