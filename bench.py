@@ -438,6 +438,17 @@ def main():
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = float(line.split(":")[1])
                 traffic_src = f"rocprofv3 PMC, {os.path.relpath(prof, ROOT)}"
+    # executed FP64 VALU flops and matrix-core busy fraction of this form (rocprofv3 SQ counters;
+    # quoted from the committed summary, which gives the command)
+    executed, mfma_busy, counters_src = None, None, None
+    cprof = os.path.join(ROOT, "profiles", f"k_batched_ccc{args.updates}_nb{NB}_valu_mfma_counters.md")
+    if os.path.exists(cprof):
+        for line in open(cprof):
+            if line.startswith(f"{best}.valu_fp64_flop_per_launch:"):
+                executed = float(line.split(":")[1])
+                counters_src = f"rocprofv3 PMC, {os.path.relpath(cprof, ROOT)}"
+            elif line.startswith(f"{best}.mfma_busy_frac:"):
+                mfma_busy = float(line.split(":")[1])
     describe = {
         "fp64_exact": "batched FP64, exact core, persistent kernel",
         "fp64_learned": f"batched FP64, core presized by the BF16 rank MLP (margin {LEARNED_MARGIN}, accepted when "
@@ -496,6 +507,10 @@ def main():
             "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 5),
             "traffic": traffic,
             "traffic_source": traffic_src,
+            "executed_fp64_flop_per_launch": executed,
+            "executed_fp64_tflops": round(executed / (hb["kernel_ms"] * 1e-3) / 1e12, 4) if executed else None,
+            "mfma_busy_frac": mfma_busy,
+            "counters_source": counters_src,
             "kernel": "k_batched (persistent batched recompression)",
             "kernel_ms": round(hb["kernel_ms"], 3),
             "alg_flops_per_launch": flops,
