@@ -507,6 +507,7 @@ def main():
             "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 5),
             "traffic": traffic,
             "traffic_source": traffic_src,
+            "traffic_gb_per_s": round(traffic / (hb["kernel_ms"] * 1e-3) / 1e9, 1) if traffic else None,
             "executed_fp64_flop_per_launch": executed,
             "executed_fp64_tflops": round(executed / (hb["kernel_ms"] * 1e-3) / 1e12, 4) if executed else None,
             "mfma_busy_frac": mfma_busy,
