@@ -26,18 +26,23 @@
 namespace hcamd {
 
 // Bits of the rotation summary the Jacobi rounds and tiles return: some pair was rotated (the
-// tile must be written back), and some rotation was above the small-angle bound (the sweep has
-// not converged).
+// tile must be written back), and some rotated pair was far from orthogonal (the sweep has not
+// converged).
 enum : int { JROT_APPLIED = 1, JROT_LARGE = 2 };
 
+// Convergence: a sweep is the last when every pair it rotated had, before its rotation, a
+// cosine |a_p . a_q| / (|a_p| |a_q|) of at most JacobiStopCos.  The one-sided Jacobi converges
+// quadratically, so the cosines such a sweep leaves are of the order of its square, and the
+// column norms (the singular values) are exact to rounding; the pairs' angles are not used
+// because two columns of nearly equal norm take a large rotation even when nearly orthogonal.
 template <typename T>
-struct JacobiSmallAngle;
+struct JacobiStopCos;
 template <>
-struct JacobiSmallAngle<double> {
-    static constexpr double value = 1.4901161193847656e-09;  // 0.1 * sqrt(2^-52)
+struct JacobiStopCos<double> {
+    static constexpr double value = 1e-5;
 };
 template <>
-struct JacobiSmallAngle<float> {
+struct JacobiStopCos<float> {
     static constexpr float value = 3.4526698e-05f;  // 0.1 * sqrt(2^-23)
 };
 
@@ -75,10 +80,12 @@ __host__ __device__ __forceinline__ constexpr int jpair_q(int rd, int k) {
 // zeta = (aqq - app) / (2 apq), written as t = sign(d) e / (|d| + hypot(d, e)) for d = aqq - app,
 // e = 2 apq (one division and one square root); c = 1/sqrt(1 + t^2), s = c t.
 template <typename T>
-__device__ __forceinline__ void jacobi_rotation(T app, T aqq, T apq, T tol, T floor2, T &c, T &s) {
+__device__ __forceinline__ void jacobi_rotation(T app, T aqq, T apq, T tol, T floor2, T &c, T &s, int &large) {
     c = T(1);
     s = T(0);
+    large = 0;
     if (app > floor2 && aqq > floor2 && fabs(apq) > tol * sqrt(app * aqq)) {
+        large = fabs(apq) > JacobiStopCos<T>::value * sqrt(app * aqq);
         const T d = aqq - app, e = T(2) * apq;
         // scale by a power of two so that d*d + e*e neither overflows nor underflows
         const int ex = ilogb(fmax(fabs(d), fabs(e)));
@@ -119,7 +126,9 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], 
     const T aqq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 1), WAVE);
     const T apq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 2), WAVE);
     T c, s;
-    jacobi_rotation(app, aqq, apq, tol, floor2, c, s);
+    int lg;
+    jacobi_rotation(app, aqq, apq, tol, floor2, c, s, lg);
+    const unsigned long long lgm = __ballot(l < NP && lg);
     int any = 0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -139,7 +148,7 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], 
                 vt[r][p] = ck * va - sk * vb;
                 vt[r][q] = sk * va + ck * vb;
             }
-            any |= JROT_APPLIED | (fabs(sk) > JacobiSmallAngle<T>::value ? JROT_LARGE : 0);
+            any |= JROT_APPLIED | (((lgm >> k) & 1ull) ? JROT_LARGE : 0);
         }
     }
     return any;
@@ -294,6 +303,7 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
     if (l == multi_sum_src_lane<NV>(idx)) buf[w * NV + idx] = tot;
     __syncthreads();
     T c = T(1), s = T(0);
+    int lg = 0;
     if (l < NP) {
         T app = T(0), aqq = T(0), apq = T(0);
 #pragma unroll
@@ -302,8 +312,9 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
             aqq += buf[w2 * NV + 3 * l + 1];
             apq += buf[w2 * NV + 3 * l + 2];
         }
-        jacobi_rotation(app, aqq, apq, tol, floor2, c, s);
+        jacobi_rotation(app, aqq, apq, tol, floor2, c, s, lg);
     }
+    const unsigned long long lgm = __ballot(lg);
     int any = 0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -323,7 +334,7 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
                 vt[r][p] = ck * va - sk * vb;
                 vt[r][q] = sk * va + ck * vb;
             }
-            any |= JROT_APPLIED | (fabs(sk) > JacobiSmallAngle<T>::value ? JROT_LARGE : 0);
+            any |= JROT_APPLIED | (((lgm >> k) & 1ull) ? JROT_LARGE : 0);
         }
     }
     return any;
@@ -426,6 +437,7 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
     if (l == multi_sum_src_lane<NV>(idx)) buf[w * NV + idx] = tot;
     __syncthreads();
     T c = T(1), s = T(0);
+    int lg = 0;
     if (l < NP) {
         T app = T(0), aqq = T(0), apq = T(0);
 #pragma unroll
@@ -434,12 +446,13 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
             aqq += buf[w2 * NV + 3 * l + 1];
             apq += buf[w2 * NV + 3 * l + 2];
         }
-        jacobi_rotation(app, aqq, apq, tol, floor2, c, s);
+        jacobi_rotation(app, aqq, apq, tol, floor2, c, s, lg);
         if (w == 0) {
             cs[2 * (rd * NP + l)] = c;
             cs[2 * (rd * NP + l) + 1] = s;
         }
     }
+    const unsigned long long lgm = __ballot(lg);
     int any = 0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -453,7 +466,7 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
                 y[r][p] = ck * a - sk * b;
                 y[r][q] = sk * a + ck * b;
             }
-            any |= JROT_APPLIED | (fabs(sk) > JacobiSmallAngle<T>::value ? JROT_LARGE : 0);
+            any |= JROT_APPLIED | (((lgm >> k) & 1ull) ? JROT_LARGE : 0);
         }
     }
     return any;
