@@ -15,10 +15,11 @@
 // Convergence test: |z_p . z_q| <= sqrt(rows) * eps * |z_p| |z_q| (relative,
 // as LAPACK's one-sided Jacobi), pairs involving a column whose norm is below
 // eps * ||J||_F are not rotated (those columns carry rounding-level
-// singular values).  A sweep ends the iteration when none of its rotations
-// has |sin| above 0.1 sqrt(eps): Jacobi converges quadratically, so the pairs'
-// remaining cosines are then of order ncols * (0.1 sqrt(eps))^2, below the
-// tolerance, and the sweep that would only confirm it is skipped.
+// singular values).  A sweep ends the iteration when every pair it rotated had
+// a cosine of at most JacobiStopCos before its rotation: Jacobi converges
+// quadratically, so the cosines it leaves are of the order of that bound's
+// square (U orthonormal to ~1e-11, singular values exact to rounding), and
+// the sweeps that would only shrink them further are skipped.
 #pragma once
 
 #include "device_common.hpp"
