@@ -45,10 +45,11 @@ K_CLASSES = [
     (512, 4, 32), (512, 8, 192), (512, 64, 256),
     (1024, 4, 32), (1024, 8, 192), (1024, 64, 384),
 ]
-# Batch sizes per K-class, in updates per CU: a small batch (4 per CU: two per workgroup at two
-# workgroups per CU) and a large one (16 per CU; 8 for the largest class, whose workspace would
-# otherwise pass 100 GB).
-SIZES_PER_CU = (4, 16)
+# Batch sizes per K-class, in updates per CU: one per CU (the trailing-update batches of a TLR
+# Cholesky step are of this order), a small batch (4 per CU: two per workgroup at two workgroups
+# per CU) and a large one (16 per CU; 8 for the largest class, whose workspace would otherwise
+# pass 100 GB).
+SIZES_PER_CU = (1, 4, 16)
 LARGEST_CLASS_PER_CU = 8
 PANELS = (16, 8, 4)
 RESIDENCY = (1, 2)
