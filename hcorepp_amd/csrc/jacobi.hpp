@@ -18,7 +18,7 @@
 // singular values).  A sweep ends the iteration when every pair it rotated had
 // a cosine of at most JacobiStopCos before its rotation: Jacobi converges
 // quadratically, so the cosines it leaves are of the order of that bound's
-// square (U orthonormal to ~1e-11, singular values exact to rounding), and
+// square (U orthonormal to within 1e-10, singular values exact to rounding), and
 // the sweeps that would only shrink them further are skipped.
 #pragma once
 
