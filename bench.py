@@ -78,37 +78,59 @@ def alg_bytes_kind(kind, nb, ra, rb, rc, r):
 
 
 TLR_N, TLR_NB, TLR_ELL, TLR_ACC, TLR_SEED = 16384, 1024, 0.2, 1e-10, 20261203
+# C tiles of the nb=1024 TLR GEMM sample: a 4 x 4 block of off-diagonal tiles, 16 updates each
+TLR_PICKS = [(i, j) for i in range(4) for j in range(8, 12)]
 
 
-def plain_tlr_leg(H, ctx, n_ctiles):
-    """Plain FP64 form, one update per launch sequence, on a 16 x 16-tile TLR covariance GEMM
+def host_info():
+    """The CPU the baselines run on: logical CPUs, the affinity mask, model, thread settings."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "MKL_NUM_THREADS": os.environ.get("MKL_NUM_THREADS")}
+
+
+def cpu_threads():
+    """Threads the CPU baselines use: the affinity mask, capped at OMP_NUM_THREADS when it is set
+    (the GPU host's CPU share for one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def tlr_tiles_dev(H, M, keys):
+    out = {}
+    for key in keys:
+        t = M.tile(*key)
+        out[key] = H.DenseTile(t[1]) if t[0] == "D" else H.CompressedTile(t[1], t[2], max_rank=t[3])
+    return out
+
+
+def plain_tlr_leg(H, ctx, M, picks):
+    """Plain FP64 form, one update per launch sequence, on the 16 x 16-tile TLR covariance GEMM
     C += A B with nb = 1024 (A = B = C = exp(-r/0.2) over 16384 points in the unit cube, diagonal
     tiles dense, off-diagonal compressed at 1e-10, maxRank 341).  A sample of C tiles receives its
     16 updates in k order (14 CCC, 1 DCC, 1 CDC each)."""
-    from hcorepp_amd.workloads import TLRMatrix
     t0 = time.time()
-    M = TLRMatrix(TLR_N, TLR_NB, 3, TLR_ELL, TLR_ACC, TLR_SEED)
-    picks = [(0, 1), (9, 4), (14, 15), (3, 12)][:n_ctiles]
     params = H.CompressionParameters(TLR_ACC)
-    dev = {}
-
-    def devtile(i, j):
-        if (i, j) not in dev:
-            t = M.tile(i, j)
-            dev[(i, j)] = H.DenseTile(t[1]) if t[0] == "D" else H.CompressedTile(t[1], t[2], max_rank=t[3])
-        return dev[(i, j)]
-
+    keys = sorted({(i, k) for i, _ in picks for k in range(M.T)} | {(k, j) for _, j in picks for k in range(M.T)})
+    dev = tlr_tiles_dev(H, M, keys)
     jobs = []
     for (i, j) in picks:
         tc = M.tile(i, j)
         C = H.CompressedTile(tc[1], tc[2], max_rank=tc[3])
-        ups = []
-        for k in range(M.T):
-            A, B = devtile(i, k), devtile(k, j)
-            kind = ("D" if i == k else "C") + ("D" if k == j else "C") + "C"
-            ups.append((kind, A, B))
+        ups = [(("D" if i == k else "C") + ("D" if k == j else "C") + "C", dev[(i, k)], dev[(k, j)])
+               for k in range(M.T)]
         jobs.append(((i, j), C, ups))
-    log(f"[bench] plain TLR nb={TLR_NB}: {len(jobs)} C tiles, {sum(len(j[2]) for j in jobs)} updates, tiles built in "
+    log(f"[bench] plain TLR nb={TLR_NB}: {len(jobs)} C tiles, {sum(len(j[2]) for j in jobs)} updates, uploaded in "
         f"{time.time() - t0:.1f}s")
     # warm-up on a throw-away copy of the first C tile
     tc = M.tile(*picks[0])
@@ -133,7 +155,7 @@ def plain_tlr_leg(H, ctx, n_ctiles):
     dt = time.perf_counter() - t1
     # accuracy: each C tile against the dense FP64 result C + sum_k A(i,k) B(k,j)
     errs = []
-    for (i, j), C, ups in jobs:
+    for (i, j), C, ups in jobs[:4]:
         ref = M.dense_block(i, j).copy()
         for k in range(M.T):
             ta, tb = M.tile(i, k), M.tile(k, j)
@@ -145,13 +167,104 @@ def plain_tlr_leg(H, ctx, n_ctiles):
     out = {"updates_per_s": round(n_up / dt, 3), "updates": n_up, "seconds": round(dt, 3),
            "fp64_tflops": round(tf, 4), "fp64_frac_of_peak": round(tf / PEAK_FP64_TFLOPS, 5),
            "alg_hbm_gb_per_s": round(byts / dt / 1e9, 2), "ranks_min_max": [int(min(ranks_seen)), int(max(ranks_seen))],
-           "rel_err_vs_dense_fp64_max": max(errs),
+           "rel_err_vs_dense_fp64_max": max(errs), "rel_err_checked_c_tiles": len(errs),
            "workload": f"{TLR_N // TLR_NB}x{TLR_N // TLR_NB}-tile TLR GEMM, nb={TLR_NB}, exp(-r/{TLR_ELL}) over "
                        f"{TLR_N} points in the unit cube (seed {TLR_SEED}), acc {TLR_ACC}, maxRank {TLR_NB // 3}; "
-                       f"C tiles {picks}, all 16 updates each"}
+                       f"{len(picks)} C tiles (rows 0-3 x cols 8-11), all 16 updates each"}
+    for b in dev.values():
+        b.buf.free()
     log(f"[bench] plain TLR nb={TLR_NB}: {n_up} updates in {dt:.2f}s -> {n_up / dt:.2f} updates/s, "
         f"{tf:.3f} TF/s FP64 ({100 * tf / PEAK_FP64_TFLOPS:.2f}% of peak), {byts / dt / 1e9:.1f} GB/s algorithmic; "
         f"ranks {min(ranks_seen)}-{max(ranks_seen)}; rel err vs dense max {max(errs):.2e}")
+    return out
+
+
+CPU_N, CPU_NB, CPU_ELL, CPU_ACC, CPU_SEED = 1024, 256, 0.1, 1e-8, 20261202
+
+
+def _orc_tile(O, t, copy=False):
+    if t[0] == "D":
+        return O.Tile.dense(t[1].copy(order="F") if copy else t[1])
+    return O.Tile.compressed(t[1], t[2], max_rank=t[3])
+
+
+def gpu_tlr4_leg(H, ctx):
+    """The 4 x 4-tile nb=256 covariance GEMM of the CPU reference-like leg through the plain FP64
+    form on the GPU (one update per launch sequence, the reference's loop order; fresh C tiles
+    uploaded before each run; one warm-up, median of 5)."""
+    from hcorepp_amd.workloads import TLRMatrix
+    M = TLRMatrix(CPU_N, CPU_NB, 2, CPU_ELL, CPU_ACC, CPU_SEED)
+    T = M.T
+    params = H.CompressionParameters(CPU_ACC)
+    dev = tlr_tiles_dev(H, M, [(i, k) for i in range(T) for k in range(T)])
+    secs = []
+    for rep in range(6):
+        Cg = tlr_tiles_dev(H, M, [(i, j) for i in range(T) for j in range(T)])
+        ctx.sync()
+        t0 = time.perf_counter()
+        for i in range(T):
+            for j in range(T):
+                for k in range(T):
+                    H.HCore.gemm(1.0, dev[(i, k)], dev[(k, j)], 1.0, Cg[(i, j)], ctx, params)
+        ctx.sync()
+        if rep:
+            secs.append(time.perf_counter() - t0)
+    sec = float(np.median(secs))
+    log(f"[bench] GPU plain 4x4 nb=256: {sec * 1e3:.1f} ms median -> {T ** 3 / sec:.1f} updates/s")
+    return {"updates_per_s": round(T ** 3 / sec, 2), "updates": T ** 3, "seconds_median": round(sec, 5),
+            "runs": "1 warm-up + median of 5", "form": "plain FP64, one update per launch sequence"}
+
+
+def cpu_tlr_legs(O, M1024, picks, threads):
+    """CPU runs of the oracle (the reference's CPU algorithm over MKL) on the tile GEMMs:
+    reference-like mode on the 4 x 4-tile nb=256 covariance GEMM (OpenMP over the 16 C tiles, MKL
+    threads = ceil(threads / 16), as omp_main.cpp:85-126; one warm-up, median of 5), and throughput
+    mode on the nb=1024 sample of the 16 x 16-tile GEMM (OpenMP over its C tiles, MKL sequential;
+    one warm-up, median of 3)."""
+    from hcorepp_amd.workloads import TLRMatrix, prefetch_tiles
+    out = {}
+    M = TLRMatrix(CPU_N, CPU_NB, 2, CPU_ELL, CPU_ACC, CPU_SEED)
+    T = M.T
+    A = [[_orc_tile(O, M.tile(i, k), copy=True) for k in range(T)] for i in range(T)]
+    B = [[_orc_tile(O, M.tile(k, j), copy=True) for j in range(T)] for k in range(T)]
+    p = O.Params(CPU_ACC)
+    mkl = max(1, -(-threads // (T * T)))
+    secs = []
+    for rep in range(6):
+        Cg = [[_orc_tile(O, M.tile(i, j), copy=True) for j in range(T)] for i in range(T)]
+        sec = O.tlr_gemm_omp(1.0, A, B, 1.0, Cg, p, threads, mkl)
+        if rep:
+            secs.append(sec)
+    sec = float(np.median(secs))
+    ranks = [Cg[i][j].rank for i in range(T) for j in range(T) if i != j]
+    out["reference_like_4x4_nb256"] = {
+        "updates_per_s": round(T ** 3 / sec, 2), "updates": T ** 3, "seconds_median": round(sec, 5),
+        "omp_threads": threads, "mkl_threads_per_tile": mkl, "runs": "1 warm-up + median of 5",
+        "final_ranks_min_max": [int(min(ranks)), int(max(ranks))],
+        "workload": f"{T}x{T}-tile TLR GEMM C += A B, nb={CPU_NB}, exp(-r/{CPU_ELL}) over {CPU_N} points in the "
+                    f"unit square (seed {CPU_SEED}), acc {CPU_ACC}, maxRank {CPU_NB // 3}, A = B = C"}
+    log(f"[bench] CPU reference-like 4x4 nb=256: {sec * 1e3:.1f} ms median -> {T ** 3 / sec:.1f} updates/s "
+        f"({threads} OpenMP threads x {mkl} MKL)")
+    # throughput mode on the nb=1024 sample
+    ii = sorted({i for i, _ in picks})
+    jj = sorted({j for _, j in picks})
+    A2 = [[_orc_tile(O, M1024.tile(i, k), copy=True) for k in range(M1024.T)] for i in ii]
+    B2 = [[_orc_tile(O, M1024.tile(k, j), copy=True) for j in jj] for k in range(M1024.T)]
+    p2 = O.Params(TLR_ACC)
+    secs = []
+    for rep in range(4):
+        Cg = [[_orc_tile(O, M1024.tile(i, j), copy=True) for j in jj] for i in ii]
+        sec = O.tlr_gemm_omp(1.0, A2, B2, 1.0, Cg, p2, threads, 1)
+        if rep:
+            secs.append(sec)
+    sec = float(np.median(secs))
+    n_up = len(ii) * len(jj) * M1024.T
+    out["throughput_16x16_nb1024_sample"] = {
+        "updates_per_s": round(n_up / sec, 3), "updates": n_up, "seconds_median": round(sec, 4),
+        "omp_threads": min(threads, len(ii) * len(jj)), "mkl_threads_per_tile": 1, "runs": "1 warm-up + median of 3",
+        "workload": f"the nb={TLR_NB} GEMM's C tiles rows {ii[0]}-{ii[-1]} x cols {jj[0]}-{jj[-1]}, 16 updates each "
+                    f"(OpenMP over the {len(ii) * len(jj)} C tiles, each tile's updates in k order)"}
+    log(f"[bench] CPU throughput nb=1024 sample: {n_up} updates in {sec:.2f}s -> {n_up / sec:.2f} updates/s")
     return out
 
 
@@ -253,8 +366,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--forms", default="fp64_exact,fp64_learned,fp64_rule,mixed",
                     help="comma-separated forms to measure (profiling runs pick one)")
-    ap.add_argument("--plain-tlr-tiles", type=int, default=2,
-                    help="C tiles of the nb=1024 TLR covariance GEMM timed in the plain form (0 = skip)")
+    ap.add_argument("--plain-tlr-tiles", type=int, default=len(TLR_PICKS),
+                    help="C tiles of the nb=1024 TLR covariance GEMM sample timed in the plain form and on the "
+                         "CPU (0 = skip)")
+    ap.add_argument("--no-cpu-tlr", action="store_true", help="skip the CPU runs of the tile GEMMs")
     ap.add_argument("--cholesky-forms", default="batched_fp64,batched_rule,batched_mixed,plain_fp64",
                     help="TLR Cholesky forms to time (empty = skip the Cholesky leg)")
     args = ap.parse_args()
@@ -383,7 +498,18 @@ def main():
     plain_full_est = args.updates / (tp / frac)
     log(f"[bench] plain FP64 (sample {ns}): {ns / tp:.1f} updates/s; work-weighted estimate over the batch "
         f"{plain_full_est:.1f} updates/s")
-    plain_tlr = plain_tlr_leg(H, ctx, args.plain_tlr_tiles) if args.plain_tlr_tiles > 0 else None
+    picks = TLR_PICKS[:args.plain_tlr_tiles]
+    M1024 = None
+    if picks:
+        from hcorepp_amd.workloads import TLRMatrix, prefetch_tiles
+        t0 = time.time()
+        M1024 = TLRMatrix(TLR_N, TLR_NB, 3, TLR_ELL, TLR_ACC, TLR_SEED)
+        keys = sorted({(i, k) for i, _ in picks for k in range(M1024.T)} | {(k, j) for _, j in picks
+                                                                             for k in range(M1024.T)} | set(picks))
+        prefetch_tiles(M1024, keys, cpu_threads())
+        log(f"[bench] nb={TLR_NB} TLR tiles: {len(keys)} built in {time.time() - t0:.1f}s")
+    plain_tlr = plain_tlr_leg(H, ctx, M1024, picks) if picks else None
+    plain_tlr4 = gpu_tlr4_leg(H, ctx) if not args.no_cpu_tlr else None
     chol_forms = [f for f in args.cholesky_forms.split(",") if f]
     if chol_forms:
         batch.close()  # free the batch workspace before the Cholesky leg
@@ -394,7 +520,7 @@ def main():
     # ---- CPU baseline (the oracle) on the whole batch; its outputs also check every form
     from oracle import oracle as O
     ncpu = min(args.cpu_sample, args.updates)
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = cpu_threads()
     oA = [O.Tile.compressed(u["au"], u["av"]) for u in ups[:ncpu]]
     oB = [O.Tile.compressed(u["bu"], u["bv"]) for u in ups[:ncpu]]
     oC = [O.Tile.compressed(u["cu"], u["cv"], max_rank=MAX_RANK) for u in ups[:ncpu]]
@@ -406,6 +532,9 @@ def main():
         np.sum([alg_flops(NB, a, b, c, 0) for a, b, c in zip(ra, rb, rc)]))
     cpu_full = args.updates / (sec / frac_cpu)
     log(f"[bench] CPU oracle ({cores} threads, {ncpu} updates): {sec:.2f}s -> {ncpu / sec:.1f} updates/s")
+    cpu_legs = None
+    if not args.no_cpu_tlr and picks:
+        cpu_legs = cpu_tlr_legs(O, M1024, picks, cores)
     orc_dense = [oC[i].to_dense() for i in range(min(nchk, ncpu))]
     for name, f in forms.items():
         errs = [np.linalg.norm(u_ @ v_ - orc_dense[i]) / np.linalg.norm(orc_dense[i])
@@ -498,6 +627,7 @@ def main():
         "rel_err_checked_updates": len(orc_dense),
         "forms": {k: form_json(k, f) for k, f in forms.items()},
         "plain_tlr_nb1024": plain_tlr,
+        "plain_tlr_4x4_nb256": plain_tlr4,
         "tlr_cholesky_nb512": chol,
         "roofline": {
             "bound": "mfma",
@@ -522,6 +652,9 @@ def main():
             "unit": "updates/s",
             "cores": cores,
             "kind": "port",
+            "host": dict(host_info(), lapack=O.lapack_path(),
+                         threads_rule="affinity mask capped at OMP_NUM_THREADS (the GPU host's CPU share)"),
+            "tile_gemm_legs": cpu_legs,
             "sample": (f"all {ncpu} updates of the batch, one pass after a {nwarm}-update warm-up; OpenMP over "
                        f"updates ({cores} threads), MKL sequential per thread ({sec:.2f} s)"
                        if ncpu == args.updates else

@@ -55,7 +55,9 @@ EXPORTED = [
     "hcorepp_ctx_create", "hcorepp_ctx_destroy", "hcorepp_ctx_sync", "hcorepp_ctx_stream",
     "hcorepp_ctx_arena_bytes", "hcorepp_ctx_compute_units", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
     "hcorepp_memcpy_async", "hcorepp_memset", "hcorepp_kernel_gemm_d", "hcorepp_compress_d", "hcorepp_compress_s",
-    "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d",
+    "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d", "hcorepp_gemm_pool_size_s",
+    "hcorepp_tile_potrf_d", "hcorepp_tile_potrf_s", "hcorepp_tile_trsm_d",
+    "hcorepp_tile_trsm_s", "hcorepp_tile_syrk_d", "hcorepp_tile_syrk_s",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
     "hcorepp_batch_workspace_bytes", "hcorepp_batch_phase_times", "hcorepp_batch_set_grid", "hcorepp_batch_get_grid",
@@ -120,8 +122,15 @@ def lib():
     L.hcorepp_gemm_s.argtypes = [P, C.c_float, C.POINTER(_Tile), C.c_char, C.POINTER(_Tile), C.c_char,
                                  C.c_float, C.POINTER(_Tile), C.POINTER(_CParams), C.c_int,
                                  C.POINTER(C.c_uint64), P, C.POINTER(I64)]
-    L.hcorepp_gemm_pool_size_d.argtypes = [C.POINTER(_Tile)] * 3 + [C.POINTER(_CParams)]
-    L.hcorepp_gemm_pool_size_d.restype = C.c_size_t
+    for t, F in (("d", D), ("s", C.c_float)):
+        getattr(L, f"hcorepp_tile_potrf_{t}").argtypes = [P, C.c_char, C.POINTER(_Tile)]
+        getattr(L, f"hcorepp_tile_trsm_{t}").argtypes = [P, C.c_char, C.c_char, C.c_char, C.c_char, F,
+                                                         C.POINTER(_Tile), C.POINTER(_Tile)]
+        getattr(L, f"hcorepp_tile_syrk_{t}").argtypes = [P, F, C.POINTER(_Tile), C.c_char, C.c_char, F,
+                                                         C.POINTER(_Tile)]
+    for t in ("d", "s"):
+        getattr(L, f"hcorepp_gemm_pool_size_{t}").argtypes = [C.POINTER(_Tile)] * 3 + [C.POINTER(_CParams)]
+        getattr(L, f"hcorepp_gemm_pool_size_{t}").restype = C.c_size_t
     L.hcorepp_batch_create.argtypes = [P, I64, C.POINTER(_Tile), C.POINTER(_Tile), C.POINTER(_Tile), D, D,
                                        C.POINTER(_CParams), C.c_int, C.POINTER(P)]
     for name in ("hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
@@ -416,6 +425,33 @@ class CompressedTile(_TileBase):
         src.free()
         return t
 
+    @classmethod
+    def from_cholesky_factors(cls, u, v_nr, max_rank=None, dtype=np.float64) -> "CompressedTile":
+        """A tile in the Cholesky-flow convention (reference src/operators/concrete/Compressed.cpp
+        aCholesky branches): A = U V^T with V stored cols x rank (ld cols) in the V slot."""
+        u = np.asarray(u, dtype=dtype)
+        v_nr = np.asarray(v_nr, dtype=dtype)
+        m, r = u.shape
+        n, r2 = v_nr.shape
+        assert r == r2
+        t = cls(np.zeros((m, 0), dtype=dtype), np.zeros((0, n), dtype=dtype),
+                max_rank=r if max_rank is None else max_rank, dtype=dtype)
+        mr = t.max_rank
+        host = np.zeros((m + n) * max(mr, 1), dtype=dtype)
+        host[: m * r] = u.reshape(-1, order="F")
+        host[m * mr: m * mr + n * r] = v_nr.reshape(-1, order="F")
+        t.buf.upload(host)
+        t.rank = r
+        return t
+
+    def cholesky_factors(self):
+        """(U, V) of a Cholesky-flow tile: U rows x rank, V cols x rank (A = U V^T)."""
+        h = self.buf.download(self.dtype)
+        u = h[: self.rows * self.rank].reshape(self.rows, self.rank, order="F").copy()
+        off = self.rows * self.max_rank
+        v = h[off: off + self.cols * self.rank].reshape(self.cols, self.rank, order="F").copy()
+        return u, v
+
     def factors(self):
         h = self.buf.download(self.dtype)
         u = h[: self.rows * self.rank].reshape(self.rows, self.rank, order="F").copy()
@@ -455,6 +491,32 @@ class HCore:
         if want_sigma:
             return flops.value, sig[: nsig.value].copy()
         return flops.value
+
+
+    @staticmethod
+    def potrf(A: "DenseTile", uplo: str, ctx: RunContext):
+        """HCore<T>::Potrf (reference src/api/HCore.cpp:586-622): LAPACK potrf of the dense tile A
+        in place, uplo 'L' or 'U' (the other strict triangle is left as it was)."""
+        t = A.c()
+        f = lib().hcorepp_tile_potrf_s if A.dtype == np.float32 else lib().hcorepp_tile_potrf_d
+        _check(f(ctx.h, uplo.encode(), C.byref(t)))
+
+    @staticmethod
+    def trsm(side: str, uplo: str, trans: str, diag: str, alpha, A: "DenseTile", B: "CompressedTile",
+             ctx: RunContext):
+        """HCore<T>::Trsm (reference src/api/HCore.cpp:624-647): BLAS trsm on the V factor of the
+        compressed B (Cholesky-flow convention: V stored cols x rank)."""
+        ta, tb = A.c(), B.c()
+        f = lib().hcorepp_tile_trsm_s if B.dtype == np.float32 else lib().hcorepp_tile_trsm_d
+        _check(f(ctx.h, side.encode(), uplo.encode(), trans.encode(), diag.encode(), alpha, C.byref(ta),
+                 C.byref(tb)))
+
+    @staticmethod
+    def syrk(alpha, A, opA: str, uplo: str, beta, Ct: "DenseTile", ctx: RunContext):
+        """HCore<T>::Syrk (reference src/api/HCore.cpp:484-584) into the dense tile Ct."""
+        ta, tc = A.c(), Ct.c()
+        f = lib().hcorepp_tile_syrk_s if Ct.dtype == np.float32 else lib().hcorepp_tile_syrk_d
+        _check(f(ctx.h, alpha, C.byref(ta), opA.encode(), uplo.encode(), beta, C.byref(tc)))
 
 
 gemm = HCore.gemm

@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "cholesky.hpp"
+#include "tileops.hpp"
 #include "autotune_table.hpp"
 
 using namespace hcamd;
@@ -40,10 +41,10 @@ hcorepp_status fail(hcorepp_status st, const char *fmt, ...) {
         if (e_ != hipSuccess) return fail(HCOREPP_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-// Dynamic LDS per workgroup for the QR / SVD / reconstruction / batched kernels:
-// 80 KB lets two workgroups share one CU's 160 KB.
-// Dynamic LDS per workgroup: two workgroups of the batched kernel share one CU's 160 KB.
-constexpr int64_t kLdsBytes = 78 * 1024;
+// Dynamic LDS per workgroup for the QR / SVD / reconstruction / batched kernels: two
+// workgroups of the batched kernel share one CU's 160 KB (the batched kernel's LDS-resident
+// Jacobi is sized by the same constant).
+constexpr int64_t kLdsBytes = BATCH_LDS_BYTES;
 
 }  // namespace
 
@@ -209,6 +210,47 @@ hcorepp_status recompress_plain(hcorepp_ctx *ctx, T alpha, const Holder<T> &ad, 
                                 hcorepp_tile *C, const hcorepp_cparams &p, uint64_t *flops, double *sigma_out,
                                 int64_t *n_sigma);
 
+// Arena bytes one plain update takes (its temporary factor products, then the recompression
+// workspace of a compressed C), the figure the pool-size query returns: a context arena of this
+// size never grows for the update.
+template <typename T>
+size_t plain_arena_need(const hcorepp_tile *A, char opA, const hcorepp_tile *B, char opB, const hcorepp_tile *C) {
+    Holder<T> ah[2], bh[2];
+    char aop[2], bop[2];
+    const int na = tile_holders<T>(A, opA, ah, aop);
+    const int nb = tile_holders<T>(B, opB, bh, bop);
+    const int total = na + nb;
+    const bool ddc = (total == 2 && C->type == HCOREPP_COMPRESSED);
+    int nt = (total == 3 || ddc) ? 1 : (total == 4 ? 2 : 0);
+    size_t need = 0;
+    int ai = na - 1, bi = 0;
+    while (nt > 0) {
+        const int64_t mm = (aop[ai] == 'T') ? ah[ai].cols : ah[ai].rows;
+        const int64_t nn = (bop[bi] == 'T') ? bh[bi].rows : bh[bi].cols;
+        need += aligned(sizeof(T) * std::max<int64_t>(mm * nn, 1));
+        Holder<T> th = {mm, nn, mm, nullptr};
+        if (nt == 2 || (nt == 1 && ai == 0)) {
+            ah[ai] = th;
+            aop[ai] = 'N';
+            bi++;
+        } else {
+            bh[bi] = th;
+            bop[bi] = 'N';
+            ai--;
+        }
+        nt--;
+    }
+    if (C->type == HCOREPP_COMPRESSED && !ddc) {
+        const int64_t arank = ah[ai].cols;
+        need += recompress_arena_bytes<T>(C->rows, C->cols, arank + C->rank, C->max_rank);
+    }
+    return need + 4096;
+}
+
+template <typename T>
+hcorepp_status gemm_cholesky(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char opA, const hcorepp_tile *B,
+                             char opB, T beta, hcorepp_tile *C, const hcorepp_cparams &p, uint64_t *flops);
+
 template <typename T>
 hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char opA, const hcorepp_tile *B,
                           char opB, T beta, hcorepp_tile *C, const hcorepp_cparams *params, int cholesky,
@@ -233,42 +275,14 @@ hcorepp_status gemm_plain(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char
     const bool ddc = (total == 2 && nc == 2);
     int ntemp = (total == 3 || ddc) ? 1 : (total == 4 ? 2 : 0);
 
-    if (cholesky)
-        return fail(HCOREPP_ERR_UNSUPPORTED,
-                    "aCholesky operand convention (reference HCore.cpp:160-174) is not in this build");
+    if (cholesky) return gemm_cholesky<T>(ctx, alpha, A, opA, B, opB, beta, C, p, flops);
     if (C->type == HCOREPP_COMPRESSED && !ddc && (opA != 'N' || opB != 'N'))
         return fail(HCOREPP_ERR_UNSUPPORTED,
                     "compressed C with transposed operands: the reference CompressedTile::Gemm ignores the ops");
 
     // ---- size the arena for this update ----
-    size_t need = 0;
-    {
-        int ai = na - 1, bi = 0, nt = ntemp;
-        Holder<T> a2[2] = {ah[0], ah[1]}, b2[2] = {bh[0], bh[1]};
-        char ao2[2] = {aop[0], aop[1]}, bo2[2] = {bop[0], bop[1]};
-        while (nt > 0) {
-            const int64_t mm = (ao2[ai] == 'T') ? a2[ai].cols : a2[ai].rows;
-            const int64_t nn = (bo2[bi] == 'T') ? b2[bi].rows : b2[bi].cols;
-            need += aligned(sizeof(T) * std::max<int64_t>(mm * nn, 1));
-            Holder<T> th = {mm, nn, mm, nullptr};
-            if (nt == 2 || (nt == 1 && ai == 0)) {
-                a2[ai] = th;
-                ao2[ai] = 'N';
-                bi++;
-            } else {
-                b2[bi] = th;
-                bo2[bi] = 'N';
-                ai--;
-            }
-            nt--;
-        }
-        if (C->type == HCOREPP_COMPRESSED && !ddc) {
-            const int64_t arank = a2[ai].cols;
-            const int64_t kmax = arank + C->rank;
-            need += recompress_arena_bytes<T>(C->rows, C->cols, kmax, C->max_rank);
-        }
-    }
-    if ((st = arena_begin(ctx, need + 4096))) return st;
+    const size_t need = plain_arena_need<T>(A, opA, B, opB, C);
+    if ((st = arena_begin(ctx, need))) return st;
 
     // ---- temporary factor products (HCore.cpp:176-264) ----
     int a_idx = na - 1, b_idx = 0;
@@ -407,6 +421,142 @@ hcorepp_status recompress_plain(hcorepp_ctx *ctx, T alpha, const Holder<T> &ad, 
     return HCOREPP_OK;
 }
 
+// Gemm in the Cholesky flow (the reference's aCholesky = true, HCore.cpp:160-174 and the aCholesky
+// branches of Compressed.cpp:273-276, 458-476, 529-595, 669-673): compressed tiles are U V^T with
+// V stored n x r (ld n), and C <- beta C + alpha A B^T for opA = 'N', opB = 'T'.  With
+// A = U_a V_a^T and B = U_b V_b^T, A B^T = U_a [(V_a^T V_b) U_b^T], so the update is the engine's
+// recompression with the A factor U_a and the B factor (V_a^T V_b) U_b^T (two small products);
+// C's V factor is transposed into the engine's r x n layout before and back after.
+template <typename T>
+hcorepp_status gemm_cholesky(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char opA, const hcorepp_tile *B,
+                             char opB, T beta, hcorepp_tile *C, const hcorepp_cparams &p, uint64_t *flops) {
+    if (A->type != HCOREPP_COMPRESSED || B->type != HCOREPP_COMPRESSED || C->type != HCOREPP_COMPRESSED ||
+        opA != 'N' || opB != 'T')
+        return fail(HCOREPP_ERR_UNSUPPORTED, "the Cholesky-flow Gemm takes compressed A, B, C with opA 'N', opB 'T' "
+                    "(the re-plumbed case of reference HCore.cpp:160-174)");
+    const int64_t m = C->rows, n = C->cols, ra = A->rank, rb = B->rank, rc = C->rank, kk = A->cols;
+    if (A->rows != m || B->rows != n || B->cols != kk)
+        return fail(HCOREPP_ERR_INVALID, "shapes: A %lldx%lld, B^T %lldx%lld, C %lldx%lld", (long long) A->rows,
+                    (long long) A->cols, (long long) B->cols, (long long) B->rows, (long long) m, (long long) n);
+    if (ra < 1 || rb < 1) return fail(HCOREPP_ERR_INVALID, "the A and B factors need rank >= 1");
+    hcorepp_status st;
+    if ((st = ensure_lds_limits())) return st;
+    const size_t need = aligned(sizeof(T) * (size_t) (ra * rb)) + aligned(sizeof(T) * (size_t) (ra * n)) +
+                        aligned(sizeof(T) * (size_t) (n * std::max<int64_t>(C->max_rank, 1))) +
+                        recompress_arena_bytes<T>(m, n, ra + rc, C->max_rank) + 4096;
+    if ((st = arena_begin(ctx, need))) return st;
+    hipStream_t s = ctx->stream;
+    T *w1 = (T *) arena_take(ctx, sizeof(T) * (size_t) (ra * rb));
+    T *w2 = (T *) arena_take(ctx, sizeof(T) * (size_t) (ra * n));
+    T *xv = (T *) arena_take(ctx, sizeof(T) * (size_t) (n * std::max<int64_t>(C->max_rank, 1)));
+    const T *ua = (const T *) A->data, *va = (const T *) A->data + A->rows * A->max_rank;
+    const T *ub = (const T *) B->data, *vb = (const T *) B->data + B->rows * B->max_rank;
+    T *cv = (T *) C->data + m * C->max_rank;
+    HIP_TRY(launch_gemm<T>(s, (int) ra, (int) rb, (int) kk, T(1), va, kk, true, vb, kk, false, T(0), w1, ra));
+    HIP_TRY(launch_gemm<T>(s, (int) ra, (int) n, (int) rb, T(1), w1, ra, false, ub, n, true, T(0), w2, ra));
+    if (flops) *flops += 2ull * ra * rb * kk + 2ull * ra * n * rb;
+    // C's V: n x rc (ld n) -> rc x n (ld rc)
+    HIP_TRY(launch_transpose<T>(s, (int) n, (int) rc, cv, n, xv, std::max<int64_t>(rc, 1)));
+    HIP_TRY(hipMemcpyAsync(cv, xv, sizeof(T) * (size_t) (rc * n), hipMemcpyDeviceToDevice, s));
+    const Holder<T> ad{m, ra, m, const_cast<T *>(ua)}, bd{ra, n, ra, w2};
+    hcorepp_tile Ce = *C;
+    st = recompress_plain<T>(ctx, alpha, ad, bd, beta, &Ce, p, flops, nullptr, nullptr);
+    const int64_t r = st ? rc : Ce.rank;  // on failure the engine left C's factors in place
+    // back to n x r (ld n)
+    HIP_TRY(launch_transpose<T>(s, (int) r, (int) n, cv, std::max<int64_t>(r, 1), xv, n));
+    HIP_TRY(hipMemcpyAsync(cv, xv, sizeof(T) * (size_t) (r * n), hipMemcpyDeviceToDevice, s));
+    if (st) return st;
+    C->rank = r;
+    C->ld = m;
+    return HCOREPP_OK;
+}
+
+// HCore<T>::Potrf (reference src/api/HCore.cpp:586-622, lapack::potrf at omp/kernels.cpp:236-243).
+template <typename T>
+hcorepp_status tile_potrf(hcorepp_ctx *ctx, char uplo, hcorepp_tile *A) {
+    hcorepp_status st;
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    if ((st = validate_tile<T>(A, "A"))) return st;
+    if (A->type != HCOREPP_DENSE) return fail(HCOREPP_ERR_INVALID, "Potrf works only with dense tiles");
+    if (A->rows != A->cols) return fail(HCOREPP_ERR_INVALID, "potrf needs a square tile");
+    if (uplo != 'L' && uplo != 'U') return fail(HCOREPP_ERR_INVALID, "uplo must be 'L' or 'U'");
+    const int64_t n = A->rows;
+    if ((st = arena_begin(ctx, aligned(sizeof(T) * (size_t) std::max<int64_t>(n * n, 1)) + 4096))) return st;
+    T *work = (T *) arena_take(ctx, sizeof(T) * (size_t) std::max<int64_t>(n * n, 1));
+    int *info = (int *) arena_take(ctx, sizeof(int));
+    HIP_TRY(launch_tile_potrf<T>(ctx->stream, uplo == 'U', (T *) A->data, (int) n, A->ld, work, info));
+    HIP_TRY(hipMemcpyAsync(ctx->pinned, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int h = *reinterpret_cast<const int *>(ctx->pinned);
+    if (h) return fail(HCOREPP_ERR_NUMERIC, "potrf: leading minor %d is not positive", h);
+    return HCOREPP_OK;
+}
+
+// HCore<T>::Trsm (reference src/api/HCore.cpp:624-647): blas::trsm on the V factor of the
+// compressed B, viewed as rows x rank with B's leading dimension.
+template <typename T>
+hcorepp_status tile_trsm(hcorepp_ctx *ctx, char side, char uplo, char trans, char diag, T alpha,
+                         const hcorepp_tile *A, hcorepp_tile *B) {
+    hcorepp_status st;
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    if ((st = validate_tile<T>(A, "A")) || (st = validate_tile<T>(B, "B"))) return st;
+    if (B->type != HCOREPP_COMPRESSED) return fail(HCOREPP_ERR_INVALID, "TRSM: Tile B must be compressed");
+    if (A->type != HCOREPP_DENSE || A->rows != A->cols) return fail(HCOREPP_ERR_INVALID, "A must be square dense");
+    if ((side != 'L' && side != 'R') || (uplo != 'L' && uplo != 'U') || (trans != 'N' && trans != 'T' &&
+        trans != 'C') || (diag != 'N' && diag != 'U'))
+        return fail(HCOREPP_ERR_INVALID, "side L/R, uplo L/U, trans N/T/C, diag N/U");
+    const int64_t m = B->rows, n = B->rank;
+    if (std::max<int64_t>(B->ld, 1) * n > B->cols * B->max_rank || B->ld < m)
+        return fail(HCOREPP_ERR_INVALID, "V viewed as %lld x %lld (ld %lld) exceeds B's V slot", (long long) m,
+                    (long long) n, (long long) B->ld);
+    if (A->rows != (side == 'L' ? m : n))
+        return fail(HCOREPP_ERR_INVALID, "A is %lldx%lld, the solve needs order %lld", (long long) A->rows,
+                    (long long) A->cols, (long long) (side == 'L' ? m : n));
+    T *v = (T *) B->data + B->rows * B->max_rank;
+    HIP_TRY(launch_tile_trsm<T>(ctx->stream, side == 'L', uplo == 'U', trans != 'N', diag == 'U', (int) m, (int) n,
+                                alpha, (const T *) A->data, A->ld, v, std::max<int64_t>(B->ld, 1)));
+    return HCOREPP_OK;
+}
+
+// HCore<T>::Syrk (reference src/api/HCore.cpp:484-584): blas::syrk on a dense A; for a
+// compressed A (U V^T, V n x r): C <- beta C - U (V^T V) U^T (alpha is not used, as in the
+// reference), then the strict triangle opposite to uplo set to zero.
+template <typename T>
+hcorepp_status tile_syrk(hcorepp_ctx *ctx, T alpha, const hcorepp_tile *A, char opA, char uplo, T beta,
+                         hcorepp_tile *C) {
+    hcorepp_status st;
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    if ((st = validate_tile<T>(A, "A")) || (st = validate_tile<T>(C, "C"))) return st;
+    if (C->type != HCOREPP_DENSE) return fail(HCOREPP_ERR_UNSUPPORTED, "syrk writes a dense C");
+    if ((opA != 'N' && opA != 'T' && opA != 'C') || (uplo != 'L' && uplo != 'U' && uplo != 'G'))
+        return fail(HCOREPP_ERR_INVALID, "opA N/T/C, uplo L/U/G");
+    hipStream_t s = ctx->stream;
+    if (C->rows != C->cols) return fail(HCOREPP_ERR_INVALID, "syrk needs a square C");
+    if (A->type == HCOREPP_DENSE) {
+        if (uplo == 'G') return fail(HCOREPP_ERR_INVALID, "syrk of a dense tile needs uplo L or U");
+        const int64_t n = C->rows, k = C->cols;
+        const int64_t arows = opA == 'N' ? n : k, acols = opA == 'N' ? k : n;
+        if (A->rows < arows || A->cols < acols) return fail(HCOREPP_ERR_INVALID, "A is smaller than op(A) n x k");
+        HIP_TRY(launch_tile_syrk_dense<T>(s, uplo == 'U', opA != 'N', (int) n, (int) k, alpha, (const T *) A->data,
+                                          A->ld, beta, (T *) C->data, C->ld));
+        return HCOREPP_OK;
+    }
+    const int64_t m = A->rows, na = A->cols, r = A->rank;
+    if (C->rows != m || C->cols != m) return fail(HCOREPP_ERR_INVALID, "C must be %lldx%lld", (long long) m,
+                                                   (long long) m);
+    if ((st = arena_begin(ctx, aligned(sizeof(T) * (size_t) std::max<int64_t>(r * r, 1)) +
+                                   aligned(sizeof(T) * (size_t) std::max<int64_t>(m * r, 1)) + 4096)))
+        return st;
+    T *w = (T *) arena_take(ctx, sizeof(T) * (size_t) std::max<int64_t>(r * r, 1));
+    T *t = (T *) arena_take(ctx, sizeof(T) * (size_t) std::max<int64_t>(m * r, 1));
+    const T *u = (const T *) A->data, *v = (const T *) A->data + m * A->max_rank;
+    HIP_TRY(launch_gemm<T>(s, (int) r, (int) r, (int) na, T(1), v, na, true, v, na, false, T(0), w, std::max<int64_t>(r, 1)));
+    HIP_TRY(launch_gemm<T>(s, (int) m, (int) r, (int) r, T(1), u, m, false, w, std::max<int64_t>(r, 1), false, T(0), t, m));
+    HIP_TRY(launch_gemm<T>(s, (int) m, (int) m, (int) r, T(-1), t, m, false, u, m, true, beta, (T *) C->data, C->ld));
+    if (uplo != 'G') HIP_TRY(launch_zero_tri<T>(s, (T *) C->data, (int) m, C->ld, uplo == 'L'));
+    return HCOREPP_OK;
+}
+
 // Compression constructor on the device: the truncated SVD of the dense tile A is the
 // recompression of A * I from an empty C, so it reuses the plain path (QR, window-pivoted QR,
 // Jacobi core SVD, the rank rule, reconstruction into the caller's tile buffer).
@@ -450,13 +600,37 @@ struct hcorepp_batch {
     int32_t *order = nullptr;
     int64_t *ranks = nullptr;
     BatchCounters *ctr = nullptr;
+    // Workspace: one slot per workgroup of the launch (the largest update's temporaries and
+    // recompression workspace), reused by every update the workgroup takes.
     char *ws = nullptr;
-    size_t ws_bytes = 0;
+    size_t slot_bytes = 0;
+    int slots = 0;
+    int64_t *hdrs = nullptr;  // HDR_WORDS header words per update, copied out of the slot
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    std::vector<int64_t> hdr_off;  // byte offset of each update's workspace header in ws
-    std::vector<char> ent_host;    // host copy of the entries (re-uploaded when the rank model changes)
+    std::vector<char> ent_host;  // host copy of the entries (re-uploaded when the rank model changes)
 };
+
+// Make room for `grid` workspace slots (drains the stream before replacing the allocation).
+static hcorepp_status ensure_slots(hcorepp_batch *b, int grid) {
+    if (grid <= b->slots) return HCOREPP_OK;
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    if (b->ws) HIP_TRY(hipFree(b->ws));
+    b->ws = nullptr;
+    b->slots = 0;
+    const size_t bytes = b->slot_bytes * (size_t) grid;
+    if (hipMalloc(&b->ws, std::max<size_t>(bytes, 256)) != hipSuccess) {
+        b->ws = nullptr;
+        return fail(HCOREPP_ERR_NOMEM, "batch workspace of %d slots x %zu bytes", grid, b->slot_bytes);
+    }
+    b->slots = grid;
+    return HCOREPP_OK;
+}
+
+// Address of header word `word` of update i (device).
+static const char *hdr_word(const hcorepp_batch *b, int64_t i, int word) {
+    return reinterpret_cast<const char *>(b->hdrs + i * HDR_WORDS + word);
+}
 
 // Autotune table row for a batch class (tallest QR, mean stacked rank, updates per CU): the
 // first row whose bounds hold.  False when the table has none (or is empty).
@@ -633,6 +807,29 @@ hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile 
     return gemm_plain<float>(ctx, alpha, A, opA, B, opB, beta, C, params, cholesky, flops, sigma_out, n_sigma);
 }
 
+hcorepp_status hcorepp_tile_potrf_d(hcorepp_ctx *ctx, char uplo, hcorepp_tile *A) {
+    return tile_potrf<double>(ctx, uplo, A);
+}
+hcorepp_status hcorepp_tile_potrf_s(hcorepp_ctx *ctx, char uplo, hcorepp_tile *A) {
+    return tile_potrf<float>(ctx, uplo, A);
+}
+hcorepp_status hcorepp_tile_trsm_d(hcorepp_ctx *ctx, char side, char uplo, char trans, char diag, double alpha,
+                                   const hcorepp_tile *A, hcorepp_tile *B) {
+    return tile_trsm<double>(ctx, side, uplo, trans, diag, alpha, A, B);
+}
+hcorepp_status hcorepp_tile_trsm_s(hcorepp_ctx *ctx, char side, char uplo, char trans, char diag, float alpha,
+                                   const hcorepp_tile *A, hcorepp_tile *B) {
+    return tile_trsm<float>(ctx, side, uplo, trans, diag, alpha, A, B);
+}
+hcorepp_status hcorepp_tile_syrk_d(hcorepp_ctx *ctx, double alpha, const hcorepp_tile *A, char opA, char uplo,
+                                   double beta, hcorepp_tile *C) {
+    return tile_syrk<double>(ctx, alpha, A, opA, uplo, beta, C);
+}
+hcorepp_status hcorepp_tile_syrk_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile *A, char opA, char uplo,
+                                   float beta, hcorepp_tile *C) {
+    return tile_syrk<float>(ctx, alpha, A, opA, uplo, beta, C);
+}
+
 hcorepp_status hcorepp_compress_d(hcorepp_ctx *ctx, int64_t rows, int64_t cols, const double *A, int64_t lda,
                                   const hcorepp_cparams *params, hcorepp_tile *out) {
     return compress_plain<double>(ctx, rows, cols, A, lda, params, out);
@@ -647,16 +844,14 @@ size_t hcorepp_gemm_pool_size_d(const hcorepp_tile *A, const hcorepp_tile *B, co
                                 const hcorepp_cparams *params) {
     (void) params;
     if (!A || !B || !C) return 0;
-    size_t total = 0;
-    const int64_t ra = A->type == HCOREPP_COMPRESSED ? A->rank : A->cols;
-    const int64_t rb = B->type == HCOREPP_COMPRESSED ? B->rank : B->rows;
-    total += (size_t) (ra * rb + ra * B->cols + A->rows * B->cols);
-    if (C->type == HCOREPP_COMPRESSED) {
-        const int64_t arank = std::max(ra, rb);
-        const WsLayout L = ws_layout<double>(C->rows, C->cols, arank + C->rank, C->max_rank);
-        total += (size_t) L.total;
-    }
-    return total;
+    return (plain_arena_need<double>(A, 'N', B, 'N', C) + sizeof(double) - 1) / sizeof(double);
+}
+
+size_t hcorepp_gemm_pool_size_s(const hcorepp_tile *A, const hcorepp_tile *B, const hcorepp_tile *C,
+                                const hcorepp_cparams *params) {
+    (void) params;
+    if (!A || !B || !C) return 0;
+    return (plain_arena_need<float>(A, 'N', B, 'N', C) + sizeof(float) - 1) / sizeof(float);
 }
 
 // ---------------------------------------------------------------------------
@@ -681,21 +876,24 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
         if (A[i].type == HCOREPP_DENSE && B[i].type == HCOREPP_DENSE)
             return fail(HCOREPP_ERR_UNSUPPORTED, "DDC updates are not batched (update %lld)", (long long) i);
     }
-    // Plan one update: fills `e` (when non-null) with pointers relative to `base`
-    // and returns the bytes of workspace the update needs.
-    auto plan = [&](int64_t i, char *base, BatchEntry<T> *e, double *cost, int64_t *rows, int64_t *kst) -> size_t {
+    // Plan one update: fills `e` (when non-null) and returns the bytes of slot workspace the update
+    // needs (its temporaries, then the recompression workspace).  Pointers into the slot are byte
+    // offsets, marked in e->rel; the kernel adds the slot's base.
+    auto plan = [&](int64_t i, BatchEntry<T> *e, double *cost, int64_t *rows, int64_t *kst) -> size_t {
         const hcorepp_tile *a = &A[i], *b = &B[i], *c = &C[i];
         Holder<T> ah[2], bh[2];
         char aop[2], bop[2];
+        bool atmp[2] = {false, false}, btmp[2] = {false, false};  // holder lives in the slot
         const int na = tile_holders<T>(a, 'N', ah, aop);
         const int nb = tile_holders<T>(b, 'N', bh, bop);
         int ntemp = (na + nb == 3) ? 1 : 2;
         int a_idx = na - 1, b_idx = 0, g = 0;
+        int32_t rel = 0;
         size_t off = 0;
         while (ntemp > 0) {
             const Holder<T> ad = ah[a_idx], bd = bh[b_idx];
             const int64_t mm = ad.rows, nn = bd.cols, kk = std::min(ad.cols, bd.rows);
-            T *dst = (T *) (base + off);
+            T *dst = reinterpret_cast<T *>((uintptr_t) off);
             if (e) {
                 GemmSpec<T> &gs = e->gemm[g];
                 gs.a = ad.data;
@@ -708,15 +906,19 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
                 gs.k = (int32_t) kk;
                 gs.ta = gs.tb = 0;
                 gs.c = dst;
+                rel |= (atmp[a_idx] ? (g ? REL_G1A : REL_G0A) : 0) | (btmp[b_idx] ? (g ? REL_G1B : REL_G0B) : 0) |
+                       (g ? REL_G1C : REL_G0C);
             }
             g++;
             off += aligned(sizeof(T) * std::max<int64_t>(mm * nn, 1));
             const Holder<T> th = {mm, nn, mm, dst};
             if (ntemp == 2 || (ntemp == 1 && a_idx == 0)) {
                 ah[a_idx] = th;
+                atmp[a_idx] = true;
                 b_idx++;
             } else {
                 bh[b_idx] = th;
+                btmp[b_idx] = true;
                 a_idx--;
             }
             ntemp--;
@@ -733,7 +935,9 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
             d.bfin = bd.data;
             d.ldbf = arank;
             d.cdata = (T *) c->data;
-            d.ws = (T *) (base + off);
+            d.ws = reinterpret_cast<T *>((uintptr_t) off);
+            rel |= (atmp[a_idx] ? REL_AFIN : 0) | (btmp[b_idx] ? REL_BFIN : 0) | REL_WS;
+            e->rel = rel;
             d.m = (int32_t) m;
             d.n = (int32_t) nn;
             d.arank = (int32_t) arank;
@@ -759,17 +963,19 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
         if (e && (ad.rows != m || bd.cols != nn || bd.rows != arank)) return 0;
         return off + aligned(sizeof(T) * L.total);
     };
-    std::vector<size_t> ws_off((size_t) n);
+    size_t slot_bytes = 256;
     std::vector<double> cost((size_t) n);
-    size_t ws_total = 0;
     int64_t max_rows = 1;
     double sum_k = 0.0;
     for (int64_t i = 0; i < n; ++i) {
         int64_t rows = 0, kst = 0;
-        ws_off[(size_t) i] = ws_total;
-        ws_total += plan(i, nullptr, nullptr, &cost[(size_t) i], &rows, &kst);
+        slot_bytes = std::max(slot_bytes, aligned(plan(i, nullptr, &cost[(size_t) i], &rows, &kst)));
         max_rows = std::max(max_rows, rows);
         sum_k += (double) kst;
+        const int64_t mu = std::min<int64_t>(std::max<int64_t>(C[i].rows, 0), kst);
+        if (mu > JACOBI_ROWS_MAX)
+            return fail(HCOREPP_ERR_UNSUPPORTED, "update %lld: stacked rank %lld exceeds the core SVD limit %d",
+                        (long long) i, (long long) kst, JACOBI_ROWS_MAX);
     }
     const double mean_k = n > 0 ? sum_k / (double) n : 0.0;
     const int64_t lds_elems = kLdsBytes / (int64_t) sizeof(T);
@@ -793,14 +999,15 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     bt->n = n;
     bt->pb = pb;
     bt->max_rows = max_rows;
-    bt->ws_bytes = ws_total;
+    bt->slot_bytes = slot_bytes;
     bt->grid = (int) std::max<int64_t>(1, std::min<int64_t>(n, (int64_t) std::max(ctx->num_cus, 1) * wg_per_cu));
     auto cleanup = [&](hcorepp_status s2) {
         hcorepp_batch_destroy(bt);
         return s2;
     };
-    if (hipMalloc(&bt->ws, std::max<size_t>(ws_total, 256)) != hipSuccess)
-        return cleanup(fail(HCOREPP_ERR_NOMEM, "batch workspace of %zu bytes", ws_total));
+    if ((st = ensure_slots(bt, bt->grid))) return cleanup(st);
+    if (hipMalloc(&bt->hdrs, sizeof(int64_t) * HDR_WORDS * (size_t) std::max<int64_t>(n, 1)) != hipSuccess)
+        return cleanup(fail(HCOREPP_ERR_NOMEM, "batch headers"));
     if (hipMalloc(&bt->entries, sizeof(BatchEntry<T>) * (size_t) std::max<int64_t>(n, 1)) != hipSuccess ||
         hipMalloc(&bt->order, sizeof(int32_t) * (size_t) std::max<int64_t>(n, 1)) != hipSuccess ||
         hipMalloc(&bt->ranks, sizeof(int64_t) * (size_t) std::max<int64_t>(n, 1)) != hipSuccess ||
@@ -811,12 +1018,10 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     for (int64_t i = 0; i < n; ++i) {
         BatchEntry<T> &e = ent[(size_t) i];
         memset(&e, 0, sizeof(e));
-        if (plan(i, bt->ws + ws_off[(size_t) i], &e, nullptr, nullptr, nullptr) == 0)
+        if (plan(i, &e, nullptr, nullptr, nullptr) == 0)
             return cleanup(fail(HCOREPP_ERR_INVALID, "operand shapes do not match C (update %lld)", (long long) i));
         e.rd.crank = bt->ranks + i;
         ranks[(size_t) i] = C[i].rank;
-        bt->hdr_off.push_back((int64_t) ((char *) e.rd.ws - bt->ws) +
-                              (int64_t) ws_layout<T>(e.rd.m, e.rd.n, e.rd.kmax, e.rd.max_rank).hdr * (int64_t) sizeof(T));
     }
     bt->ent_host.assign((const char *) ent.data(), (const char *) ent.data() + sizeof(BatchEntry<T>) * (size_t) n);
     std::vector<int32_t> order((size_t) n);
@@ -824,7 +1029,9 @@ hcorepp_status hcorepp_batch_create(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
     hipStream_t s = ctx->stream;
     // counters start at zero, so that status reads before the first pass (or of an empty batch) are clean
-    if (launch_reset_counters(s, bt->ctr) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    if (launch_reset_counters(s, bt->ctr) != hipSuccess ||
+        hipMemsetAsync(bt->hdrs, 0, sizeof(int64_t) * HDR_WORDS * (size_t) std::max<int64_t>(n, 1), s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return cleanup(fail(HCOREPP_ERR_HIP, "batch counter reset"));
     if (n > 0) {
         if (hipMemcpyAsync(bt->entries, ent.data(), sizeof(BatchEntry<T>) * n, hipMemcpyHostToDevice, s) !=
@@ -843,15 +1050,20 @@ hcorepp_status hcorepp_batch_run(hcorepp_batch *b) {
     if (b->n == 0) return HCOREPP_OK;
     hipStream_t s = b->ctx->stream;
     HIP_TRY(launch_reset_counters(s, b->ctr));
+    if (b->grid > b->slots) return fail(HCOREPP_ERR_INVALID, "grid %d exceeds the %d workspace slots", b->grid, b->slots);
     HIP_TRY(launch_batched<double>(s, (const BatchEntry<double> *) b->entries, b->order, b->n, b->ctr, b->grid,
-                                   kLdsBytes, b->pb, b->precision == HCOREPP_PREC_MIXED ? 1 : 0));
+                                   kLdsBytes, b->pb, b->precision == HCOREPP_PREC_MIXED ? 1 : 0, b->ws,
+                                   (int64_t) b->slot_bytes, b->hdrs));
     return HCOREPP_OK;
 }
 
 hcorepp_status hcorepp_batch_set_grid(hcorepp_batch *b, int workgroups) {
     if (!b || workgroups < 1) return fail(HCOREPP_ERR_INVALID, "bad arguments");
-    b->grid = (int) std::min<int64_t>(workgroups, std::max<int64_t>(b->n, 1));
+    const int grid = (int) std::min<int64_t>(workgroups, std::max<int64_t>(b->n, 1));
     drop_graph(b);
+    hcorepp_status st = ensure_slots(b, grid);
+    if (st) return st;
+    b->grid = grid;
     return HCOREPP_OK;
 }
 
@@ -883,8 +1095,7 @@ hcorepp_status hcorepp_batch_phase_times(hcorepp_batch *b, int64_t *times) {
 #else
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     for (int64_t i = 0; i < b->n; ++i)
-        HIP_TRY(hipMemcpy(times + 16 * i, b->ws + b->hdr_off[(size_t) i] + HDR_T0 * sizeof(int64_t),
-                          16 * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(times + 16 * i, hdr_word(b, i, HDR_T0), 16 * sizeof(int64_t), hipMemcpyDeviceToHost));
     return HCOREPP_OK;
 #endif
 }
@@ -941,7 +1152,9 @@ hcorepp_status hcorepp_batch_status(hcorepp_batch *b, int64_t *max_sweeps) {
     return HCOREPP_OK;
 }
 
-size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b) { return b ? b->ws_bytes : 0; }
+size_t hcorepp_batch_workspace_bytes(hcorepp_batch *b) {
+    return b ? b->slot_bytes * (size_t) b->slots + sizeof(int64_t) * HDR_WORDS * (size_t) b->n : 0;
+}
 
 hcorepp_status hcorepp_batch_set_rank_model(hcorepp_batch *b, const uint16_t *model_bf16, double tol, int margin) {
     if (!b || tol < 0.0 || (model_bf16 && (!(tol > 0.0) || margin < 0))) return fail(HCOREPP_ERR_INVALID, "bad arguments");
@@ -963,7 +1176,7 @@ hcorepp_status hcorepp_batch_features(hcorepp_batch *b, float *features) {
     if (!b || (!features && b->n > 0)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     for (int64_t i = 0; i < b->n; ++i)
-        HIP_TRY(hipMemcpy(features + (size_t) RANK_FEATURES * i, b->ws + b->hdr_off[(size_t) i] + HDR_FEAT * 8,
+        HIP_TRY(hipMemcpy(features + (size_t) RANK_FEATURES * i, hdr_word(b, i, HDR_FEAT),
                           sizeof(float) * RANK_FEATURES, hipMemcpyDeviceToHost));
     return HCOREPP_OK;
 }
@@ -976,8 +1189,7 @@ hcorepp_status hcorepp_batch_learned_stats(hcorepp_batch *b, int64_t *fallbacks,
     if (fallbacks) *fallbacks = (int64_t) c->fallbacks;
     if (predicted) {
         for (int64_t i = 0; i < b->n; ++i)
-            HIP_TRY(hipMemcpy(predicted + i, b->ws + b->hdr_off[(size_t) i] + HDR_PRED * 8, sizeof(int64_t),
-                              hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(predicted + i, hdr_word(b, i, HDR_PRED), sizeof(int64_t), hipMemcpyDeviceToHost));
     }
     return HCOREPP_OK;
 }
@@ -986,8 +1198,7 @@ hcorepp_status hcorepp_batch_core_widths(hcorepp_batch *b, int64_t *widths) {
     if (!b || (b->n > 0 && !widths)) return fail(HCOREPP_ERR_INVALID, "bad arguments");
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     for (int64_t i = 0; i < b->n; ++i)
-        HIP_TRY(hipMemcpy(widths + i, b->ws + b->hdr_off[(size_t) i] + HDR_KT * 8, sizeof(int64_t),
-                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(widths + i, hdr_word(b, i, HDR_KT), sizeof(int64_t), hipMemcpyDeviceToHost));
     return HCOREPP_OK;
 }
 
@@ -1148,6 +1359,7 @@ hcorepp_status hcorepp_batch_destroy(hcorepp_batch *b) {
     if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->ws) hipFree(b->ws);
+    if (b->hdrs) hipFree(b->hdrs);
     if (b->entries) hipFree(b->entries);
     if (b->order) hipFree(b->order);
     if (b->ranks) hipFree(b->ranks);

@@ -22,29 +22,29 @@ constexpr int ROWS_TRSM_LDS = RB * (RB + 1) + RB * NT;
 // pieces; the diagonal block is then solved by forward substitution in LDS (the thread's RB
 // values in column-interleaved slots, so that loops stay rolled).  `lds` holds ROWS_TRSM_LDS
 // doubles.  Every thread of the workgroup must call it (it synchronises).
-__device__ __forceinline__ void rows_trsm_lt(double *X, int rows, int n, int64_t ldx, const double *L, int64_t ldl,
-                                             double *lds) {
-    double *lt = lds;                   // RB x (RB + 1)
-    double *xs = lds + RB * (RB + 1);   // RB x NT
+template <typename T>
+__device__ __forceinline__ void rows_trsm_lt(T *X, int rows, int n, int64_t ldx, const T *L, int64_t ldl, T *lds) {
+    T *lt = lds;                   // RB x (RB + 1)
+    T *xs = lds + RB * (RB + 1);   // RB x NT
     const int tid = (int) threadIdx.x;
     for (int rbase = 0; rbase < rows; rbase += NT) {
         const int a = rbase + tid;
         const bool act = a < rows;
         for (int j0 = 0; j0 < n; j0 += RB) {
             const int jb = (n - j0) < RB ? (n - j0) : RB;
-            double acc[RB];
+            T acc[RB];
 #pragma unroll
-            for (int c = 0; c < RB; ++c) acc[c] = (act && c < jb) ? X[a + (int64_t) (j0 + c) * ldx] : 0.0;
+            for (int c = 0; c < RB; ++c) acc[c] = (act && c < jb) ? X[a + (int64_t) (j0 + c) * ldx] : T(0);
             for (int k0 = 0; k0 < j0; k0 += RB) {
                 __syncthreads();
                 for (int e = tid; e < RB * RB; e += NT) {
                     const int c = e % RB, kk = e / RB;
-                    lt[c * (RB + 1) + kk] = c < jb ? L[(j0 + c) + (int64_t) (k0 + kk) * ldl] : 0.0;
+                    lt[c * (RB + 1) + kk] = c < jb ? L[(j0 + c) + (int64_t) (k0 + kk) * ldl] : T(0);
                 }
                 __syncthreads();
                 if (act) {
                     for (int kk = 0; kk < RB; ++kk) {
-                        const double x = X[a + (int64_t) (k0 + kk) * ldx];
+                        const T x = X[a + (int64_t) (k0 + kk) * ldx];
 #pragma unroll
                         for (int c = 0; c < RB; ++c) acc[c] -= x * lt[c * (RB + 1) + kk];
                     }
@@ -53,14 +53,14 @@ __device__ __forceinline__ void rows_trsm_lt(double *X, int rows, int n, int64_t
             __syncthreads();
             for (int e = tid; e < RB * RB; e += NT) {
                 const int c = e % RB, cc = e / RB;
-                lt[c * (RB + 1) + cc] = (c < jb && cc < jb) ? L[(j0 + c) + (int64_t) (j0 + cc) * ldl] : 1.0;
+                lt[c * (RB + 1) + cc] = (c < jb && cc < jb) ? L[(j0 + c) + (int64_t) (j0 + cc) * ldl] : T(1);
             }
 #pragma unroll
             for (int c = 0; c < RB; ++c) xs[c * NT + tid] = acc[c];
             __syncthreads();
             if (act) {
                 for (int c = 0; c < jb; ++c) {
-                    double x = xs[c * NT + tid];
+                    T x = xs[c * NT + tid];
                     for (int cc = 0; cc < c; ++cc) x -= xs[cc * NT + tid] * lt[c * (RB + 1) + cc];
                     x /= lt[c * (RB + 1) + c];
                     xs[c * NT + tid] = x;
@@ -74,8 +74,9 @@ __device__ __forceinline__ void rows_trsm_lt(double *X, int rows, int n, int64_t
 // Diagonal block of the blocked potrf: unblocked Cholesky of A[j0:j0+jb, j0:j0+jb] in LDS.
 // info[0] receives the first non-positive pivot (1-based, whole-tile numbering) unless already set;
 // such a pivot is replaced by 1 so that the factorisation runs to the end.
-__global__ void HCAMD_KERNEL k_potrf_diag(double *A, int64_t lda, int j0, int jb, int *info) {
-    __shared__ double d[CB * (CB + 1)];
+template <typename T>
+__global__ void HCAMD_KERNEL k_potrf_diag(T *A, int64_t lda, int j0, int jb, int *info) {
+    __shared__ T d[CB * (CB + 1)];
     for (int e = threadIdx.x; e < jb * jb; e += NT) {
         const int i = e % jb, c = e / jb;
         d[i + c * (CB + 1)] = A[(j0 + i) + (int64_t) (j0 + c) * lda];
@@ -83,14 +84,14 @@ __global__ void HCAMD_KERNEL k_potrf_diag(double *A, int64_t lda, int j0, int jb
     __syncthreads();
     for (int j = 0; j < jb; ++j) {
         if (threadIdx.x == 0) {
-            const double v = d[j + j * (CB + 1)];
-            double p = 1.0;
-            if (v > 0.0) p = sqrt(v);
+            const T v = d[j + j * (CB + 1)];
+            T p = T(1);
+            if (v > T(0)) p = sqrt(v);
             else if (info[0] == 0) info[0] = j0 + j + 1;
             d[j + j * (CB + 1)] = p;
         }
         __syncthreads();
-        const double p = d[j + j * (CB + 1)];
+        const T p = d[j + j * (CB + 1)];
         for (int i = j + 1 + threadIdx.x; i < jb; i += NT) d[i + j * (CB + 1)] /= p;
         __syncthreads();
         for (int e = threadIdx.x; e < jb * jb; e += NT) {
@@ -107,18 +108,20 @@ __global__ void HCAMD_KERNEL k_potrf_diag(double *A, int64_t lda, int j0, int jb
 
 // Sub-diagonal panel of the blocked potrf: A[j0+jb:n, j0:j0+jb] <- (that) L_jj^{-T}, rows dealt to
 // workgroups NT at a time.
-__global__ void HCAMD_KERNEL k_potrf_panel(double *A, int n, int64_t lda, int j0, int jb) {
-    __shared__ double lds[ROWS_TRSM_LDS];
+template <typename T>
+__global__ void HCAMD_KERNEL k_potrf_panel(T *A, int n, int64_t lda, int j0, int jb) {
+    __shared__ T lds[ROWS_TRSM_LDS];
     const int r0 = j0 + jb + (int) blockIdx.x * NT;
     const int rows = (n - r0) < NT ? (n - r0) : NT;
     if (rows <= 0) return;
     rows_trsm_lt(A + r0 + (int64_t) j0 * lda, rows, jb, lda, A + j0 + (int64_t) j0 * lda, lda, lds);
 }
 
-__global__ void HCAMD_KERNEL k_zero_upper(double *A, int n, int64_t lda) {
+template <typename T>
+__global__ void HCAMD_KERNEL k_zero_upper(T *A, int n, int64_t lda) {
     for (int64_t e = (int64_t) blockIdx.x * NT + threadIdx.x; e < (int64_t) n * n; e += (int64_t) gridDim.x * NT) {
         const int i = (int) (e % n), c = (int) (e / n);
-        if (i < c) A[i + (int64_t) c * lda] = 0.0;
+        if (i < c) A[i + (int64_t) c * lda] = T(0);
     }
 }
 
@@ -130,7 +133,7 @@ __global__ void k_zero_int(int *p) {
 __global__ void HCAMD_KERNEL k_trsm_v(const CholTrsm *jobs) {
     __shared__ double lds[ROWS_TRSM_LDS];
     const CholTrsm jb = jobs[blockIdx.x];
-    rows_trsm_lt(jb.v, (int) jb.rank, (int) jb.n, jb.rank, jb.l, jb.ldl, lds);
+    rows_trsm_lt<double>(jb.v, (int) jb.rank, (int) jb.n, jb.rank, jb.l, jb.ldl, lds);
 }
 
 // A (n x n dense) <- A - U (V V^T) U^T, with scratch W (r x r) and T (n x r); the strict upper
@@ -172,26 +175,29 @@ __global__ void HCAMD_KERNEL k_transpose_factors(const CholTrans *jobs) {
 // Blocked right-looking potrf: per block column, the diagonal block (one workgroup), the panel
 // below it (row solves over the GPU), and the trailing update A22 -= L21 L21^T as a dense GEMM
 // (both triangles; the strict upper part is cleared at the end).
-hipError_t launch_potrf(hipStream_t s, double *A, int n, int64_t lda, int *info) {
+template <typename T>
+hipError_t launch_potrf(hipStream_t s, T *A, int n, int64_t lda, int *info) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, info);
     for (int j0 = 0; j0 < n; j0 += CB) {
         const int jb = (n - j0) < CB ? (n - j0) : CB;
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(NT), 0, s, A, lda, j0, jb, info);
+        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(NT), 0, s, A, lda, j0, jb, info);
         const int rest = n - j0 - jb;
         if (rest > 0) {
-            hipLaunchKernelGGL(k_potrf_panel, dim3((rest + NT - 1) / NT), dim3(NT), 0, s, A, n, lda, j0, jb);
-            double *L21 = A + (j0 + jb) + (int64_t) j0 * lda;
-            hipError_t e = launch_gemm<double>(s, rest, rest, jb, -1.0, L21, lda, false, L21, lda, true, 1.0,
-                                               A + (j0 + jb) + (int64_t) (j0 + jb) * lda, lda);
+            hipLaunchKernelGGL(k_potrf_panel<T>, dim3((rest + NT - 1) / NT), dim3(NT), 0, s, A, n, lda, j0, jb);
+            T *L21 = A + (j0 + jb) + (int64_t) j0 * lda;
+            hipError_t e = launch_gemm<T>(s, rest, rest, jb, T(-1), L21, lda, false, L21, lda, true, T(1),
+                                          A + (j0 + jb) + (int64_t) (j0 + jb) * lda, lda);
             if (e != hipSuccess) return e;
         }
     }
     int64_t work = (int64_t) n * n;
     int grid = (int) ((work + NT - 1) / NT);
     if (grid > 1024) grid = 1024;
-    hipLaunchKernelGGL(k_zero_upper, dim3(grid > 0 ? grid : 1), dim3(NT), 0, s, A, n, lda);
+    hipLaunchKernelGGL(k_zero_upper<T>, dim3(grid > 0 ? grid : 1), dim3(NT), 0, s, A, n, lda);
     return hipGetLastError();
 }
+template hipError_t launch_potrf<double>(hipStream_t, double *, int, int64_t, int *);
+template hipError_t launch_potrf<float>(hipStream_t, float *, int, int64_t, int *);
 hipError_t launch_trsm_v(hipStream_t s, const CholTrsm *jobs, int n) {
     if (n > 0) hipLaunchKernelGGL(k_trsm_v, dim3(n), dim3(NT), 0, s, jobs);
     return hipGetLastError();

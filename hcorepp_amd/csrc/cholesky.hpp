@@ -23,7 +23,10 @@ struct CholTrans {  // dst = src^T for compressed tiles
     int64_t m, n, rank, src_max_rank, dst_max_rank;
 };
 
-hipError_t launch_potrf(hipStream_t s, double *A, int n, int64_t lda, int *info);
+// Lower Cholesky factor in place (blocked); the strict upper part is zeroed.  info: first
+// non-positive pivot (1-based) or 0.
+template <typename T>
+hipError_t launch_potrf(hipStream_t s, T *A, int n, int64_t lda, int *info);
 hipError_t launch_trsm_v(hipStream_t s, const CholTrsm *jobs, int n);
 hipError_t launch_syrk(hipStream_t s, const CholSyrk *jobs, int n);
 hipError_t launch_transpose_factors(hipStream_t s, const CholTrans *jobs, int n);

@@ -86,6 +86,60 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
     }
 }
 
+// One 64 x 64 tile of C (M x N, ld ldc) = sum_k fa(i, k) fb(k, j) for operands given as element
+// functions (each returns zero outside its structure: triangles, gathered rows); 16-deep K
+// chunks are staged in LDS as in wg_gemm_tile.  A's chunk is read row-fastest, B's K-fastest.
+template <typename T, typename FA, typename FB>
+__device__ void wg_gemm_tile_fn(int M, int N, int Kd, const FA &fa, const FB &fb, T *C, int64_t ldc, int ti, int tj,
+                                T *lds) {
+    T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
+    T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int i0 = ti * GT_M, j0 = tj * GT_N;
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+        for (int e = threadIdx.x; e < GT_M * GT_K; e += NT) {
+            const int i = e % GT_M, k = e / GT_M;
+            const int gi = i0 + i, gk = k0 + k;
+            As[k * (GT_M + 1) + i] = (gi < M && gk < Kd) ? fa(gi, gk) : T(0);
+        }
+        for (int e = threadIdx.x; e < GT_N * GT_K; e += NT) {
+            const int k = e % GT_K, j = e / GT_K;
+            const int gj = j0 + j, gk = k0 + k;
+            Bs[k * (GT_N + 1) + j] = (gj < N && gk < Kd) ? fb(gk, gj) : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < GT_K; ++k) {
+            T a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = As[k * (GT_M + 1) + ty + 16 * q];
+                b[q] = Bs[k * (GT_N + 1) + tx + 16 * q];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int gi = i0 + ty + 16 * p;
+        if (gi >= M) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int gj = j0 + tx + 16 * q;
+            if (gj < N) C[gi + (int64_t) gj * ldc] = acc[p][q];
+        }
+    }
+}
+
 // Whole-matrix GEMM by one workgroup (loops over all 64x64 tiles).
 // `upper_b` treats op(B)(k, j) as zero for k < j (a transposed upper-triangular factor).
 template <typename T>

@@ -98,12 +98,19 @@ __device__ __forceinline__ void jacobi_rotation(T app, T aqq, T apq, T tol, T fl
     }
 }
 
+// Row count of the V register tile: RPLV = 0 runs the Jacobi without accumulating V (the
+// caller recovers the right factor from the left one), the array keeps one unused row.
+template <int RPLV>
+__host__ __device__ constexpr int vrows() {
+    return RPLV > 0 ? RPLV : 1;
+}
+
 // One round: TW/2 disjoint pairs of the register tile y.  The 3*TW/2 dot products are
 // reduce-scattered across the wave with permlane/DPP exchanges; lane k < TW/2 gathers the three
 // sums of pair k, computes its rotation, and every lane reads the rotations back with readlane.
-// The rotations are applied to the J tile y and to the matching V columns vt.
+// The rotations are applied to the J tile y and to the matching V columns vt (none for RPLV 0).
 template <typename T, int RPL, int RPLV, int TW, bool FULL>
-__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], int rd, T tol, T floor2) {
+__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], int rd, T tol, T floor2) {
     constexpr int NP = TW / 2;
     constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
     const int l = lane_id();
@@ -164,7 +171,7 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
                            T tol, T floor2) {
     constexpr int B = TW / 2;
     const int l = lane_id();
-    T y[RPL][TW], vt[RPLV][TW];
+    T y[RPL][TW], vt[vrows<RPLV>()][TW];
 #pragma unroll
     for (int c = 0; c < TW; ++c) {
         const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -218,7 +225,8 @@ __host__ __device__ constexpr int block_jacobi_lds_elems() {
 // so it may be called out of line).  Returns the sweeps used (max_sweeps when it did not
 // converge).
 template <typename T, int RPL, int RPLV, int TW>
-__device__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int max_sweeps, T *gbuf) {
+__device__ __forceinline__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv,
+                                               int max_sweeps, T *gbuf) {
     constexpr int B = TW / 2;
     T *s_red = gbuf;
     int &s_rot = *reinterpret_cast<int *>(s_red + NWAVES);

@@ -122,7 +122,7 @@ __global__ void HCAMD_KERNEL k_qr_panel(RecompDesc<T> d, int which, int j0, T *v
                               q.pivot ? cn : nullptr);
     }
     const int rows = q.m - j0;
-    for (int e = threadIdx.x; e < rows * PB; e += NT) vpan[e] = lds[e];
+    for (int e = threadIdx.x; e < panel_ld(rows) * PB; e += NT) vpan[e] = lds[e];
 }
 
 template <typename T, int PB, int RPT>
@@ -138,7 +138,8 @@ __global__ void HCAMD_KERNEL k_qr_update(RecompDesc<T> d, int which, int j0, int
     const int c0 = j0 + jb + (int) blockIdx.x * cols_per_wg;
     const int c1 = (c0 + cols_per_wg) < q.ncols ? (c0 + cols_per_wg) : q.ncols;
     if (c0 >= c1) return;
-    apply_wy<T, PB, RPT>(vpan, q.tsave + (int64_t) (j0 / PB) * PB * PB, q.m - j0, q.A + j0, q.lda, c0, c1, true, red);
+    apply_wy<T, PB, RPT>(vpan, panel_ld(q.m - j0), q.tsave + (int64_t) (j0 / PB) * PB * PB, q.m - j0, q.A + j0, q.lda,
+                         c0, c1, true, red);
     if (q.pivot) {
         // norms over rows [j0 + jb, m) of the updated columns, for the next panel's pivoting
         __syncthreads();
@@ -176,7 +177,7 @@ static hipError_t launch_qr_mw_pb(hipStream_t s, const RecompDesc<T> &d, int64_t
                                   int ncols, T *vpan) {
     constexpr int NCB = WAVE / PB;
     const int nsteps = m < ncols ? m : ncols;
-    T *cn = vpan + (int64_t) (d.m > d.n ? d.m : d.n) * 16;  // after the largest panel
+    T *cn = vpan + (int64_t) ((d.m > d.n ? d.m : d.n) + 1) * 16;  // after the largest panel (ld rows | 1)
     if (which == 1)
         hipLaunchKernelGGL((k_qr_colnorms<T, PB>), dim3((ncols + NWAVES - 1) / NWAVES), dim3(NT), 0, s, d, which, cn);
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
@@ -383,30 +384,53 @@ __device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds, int mixed) {
     }
 }
 
+// The rebased copy of one batch entry: pointers named by `rel` are byte offsets into `slot`.
+template <typename P>
+__device__ __forceinline__ P rebase(P p, int32_t rel, int32_t bit, char *slot) {
+    return (rel & bit) ? reinterpret_cast<P>(slot + reinterpret_cast<uintptr_t>(p)) : p;
+}
+
 template <typename T, int PB>
 __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32_t *order, int64_t n,
-                                                BatchCounters *ctr, int mixed) {
+                                       BatchCounters *ctr, int mixed, char *slots, int64_t slot_bytes,
+                                       int64_t *hdrs) {
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     __shared__ int64_t s_item;
+    __shared__ BatchEntry<T> s_ent;  // the current update, its pointers rebased onto this slot
+    char *slot = slots + (int64_t) blockIdx.x * slot_bytes;
     for (;;) {
         if (threadIdx.x == 0) s_item = (int64_t) atomicAdd(&ctr->next, 1ULL);
         __syncthreads();
         const int64_t item = s_item;
-        __syncthreads();
         if (item >= n) break;
-        const BatchEntry<T> &e = entries[order[item]];
+        const int32_t idx = order[item];
+        if (threadIdx.x == 0) {
+            BatchEntry<T> e = entries[idx];
+            const int32_t rel = e.rel;
+            e.gemm[0].a = rebase(e.gemm[0].a, rel, REL_G0A, slot);
+            e.gemm[0].b = rebase(e.gemm[0].b, rel, REL_G0B, slot);
+            e.gemm[0].c = rebase(e.gemm[0].c, rel, REL_G0C, slot);
+            e.gemm[1].a = rebase(e.gemm[1].a, rel, REL_G1A, slot);
+            e.gemm[1].b = rebase(e.gemm[1].b, rel, REL_G1B, slot);
+            e.gemm[1].c = rebase(e.gemm[1].c, rel, REL_G1C, slot);
+            e.rd.afin = rebase(e.rd.afin, rel, REL_AFIN, slot);
+            e.rd.bfin = rebase(e.rd.bfin, rel, REL_BFIN, slot);
+            e.rd.ws = rebase(e.rd.ws, rel, REL_WS, slot);
+            s_ent = e;
+        }
+        __syncthreads();
+        const BatchEntry<T> &e = s_ent;
         const RecompDesc<T> &d = e.rd;
-#ifdef HCAMD_PHASE_TIMING
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 8; ++k) ws_of(d).hdr[HDR_SUB + k] = 0;
-#endif
+        Ws<T> w = ws_of(d);
+        // the slot's header still holds the previous update's words: start from zero
+        for (int k = threadIdx.x; k < HDR_WORDS; k += NT) w.hdr[k] = 0;
+        __syncthreads();
         const uint64_t t_start = HCAMD_TSTAMP();
         run_temp_gemms<T>(e, lds, mixed);
         const uint64_t t_gemm = HCAMD_TSTAMP();
         ph_stack<T>(d, 0, 1);
         __syncthreads();
-        Ws<T> w = ws_of(d);
         const Dims z = dims_of(d, w);
         if (z.K <= d.kmax) ph_features<T>(d, lds);
         uint64_t t_qru = 0, t_y = 0, t_qry = 0, t_svd = 0, t_ru = 0;
@@ -424,17 +448,18 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             ph_core_build<T>(d);
             ph_core_qr<T, PB>(d, lds);
             t_qry = HCAMD_TSTAMP();
-            ph_svd<T>(d, lds);
+            ph_svd_nov<T>(d, lds);
             __syncthreads();
             t_svd = HCAMD_TSTAMP();
         }
-        const int r = (z.K <= d.kmax) ? (int) w.hdr[HDR_RANK] : (int) *d.crank;
-        if (z.K <= d.kmax) {
-            ph_recon<T, PB>(d, 0, 0, r, lds);
+        // A core beyond the Jacobi's limit, or NaN/Inf in the factors, leaves C as it was: its
+        // factors and rank are kept and the status word reports the update.
+        const bool keep_c = (z.K > d.kmax) || (w.hdr[HDR_STATUS] & (STATUS_TOO_BIG | STATUS_NAN)) != 0;
+        const int r = keep_c ? (int) *d.crank : (int) w.hdr[HDR_RANK];
+        if (!keep_c) {
+            ph_recon_nov<T, PB>(d, lds);
             __syncthreads();
             t_ru = HCAMD_TSTAMP();
-            ph_recon<T, PB>(d, 1, 0, r, lds);
-            __syncthreads();
             ph_finish<T>(d, 0, 1);
             __syncthreads();
         }
@@ -443,6 +468,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             const uint64_t ts[8] = {t_start, t_gemm, t_qru, t_y, t_qry, t_svd, t_ru, HCAMD_TSTAMP()};
             for (int k = 0; k < 8; ++k) w.hdr[HDR_T0 + k] = (int64_t) ts[k];
         }
+        __syncthreads();
 #else
         (void) t_start; (void) t_gemm; (void) t_qru; (void) t_y; (void) t_qry; (void) t_svd; (void) t_ru;
 #endif
@@ -453,6 +479,8 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             if (z.K <= d.kmax && w.hdr[HDR_LEARNED] == 2) atomicAdd(&ctr->fallbacks, 1ULL);
             atomicMax(&ctr->max_sweeps, (unsigned long long) w.hdr[HDR_SWEEPS]);
         }
+        // the header outlives the slot: diagnostics read it per update
+        for (int k = threadIdx.x; k < HDR_WORDS; k += NT) hdrs[(int64_t) idx * HDR_WORDS + k] = w.hdr[k];
         __syncthreads();
     }
 }
@@ -579,11 +607,21 @@ hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c) {
 
 template <typename T>
 hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,
-                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb, int mixed) {
+                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb, int mixed, char *slots,
+                          int64_t slot_bytes, int64_t *hdrs) {
     switch (pb) {
-        case 16: hipLaunchKernelGGL((k_batched<T, 16>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed); break;
-        case 8: hipLaunchKernelGGL((k_batched<T, 8>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed); break;
-        default: hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed); break;
+        case 16:
+            hipLaunchKernelGGL((k_batched<T, 16>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed,
+                               slots, slot_bytes, hdrs);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_batched<T, 8>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed,
+                               slots, slot_bytes, hdrs);
+            break;
+        default:
+            hipLaunchKernelGGL((k_batched<T, 4>), dim3(grid), dim3(NT), lds_bytes, s, entries, order, n, ctr, mixed,
+                               slots, slot_bytes, hdrs);
+            break;
     }
     return hipGetLastError();
 }
@@ -613,7 +651,7 @@ hipError_t set_kernel_lds_limits(int64_t lds_bytes) {
     template hipError_t launch_readjust_ddc<T>(hipStream_t, const T *, int64_t, int64_t, T *);                   \
     template hipError_t launch_fill_identity<T>(hipStream_t, T *, int64_t);                                      \
     template hipError_t launch_batched<T>(hipStream_t, const BatchEntry<T> *, const int32_t *, int64_t,          \
-                                          BatchCounters *, int, int64_t, int, int);                              \
+                                          BatchCounters *, int, int64_t, int, int, char *, int64_t, int64_t *);  \
     template hipError_t set_kernel_lds_limits<T>(int64_t);
 
 HCAMD_INSTANTIATE(double)

@@ -31,12 +31,18 @@ struct GemmSpec {
     int32_t pad;
 };
 
-// One update of a batch: up to two temporary products then the recompression.
+// One update of a batch: up to two temporary products then the recompression.  The
+// temporaries and the recompression workspace live in the slot of the workgroup that runs the
+// update: the pointers named by the bits of `rel` hold byte offsets into that slot.
+enum : int32_t {
+    REL_G0A = 1, REL_G0B = 2, REL_G0C = 4, REL_G1A = 8, REL_G1B = 16, REL_G1C = 32,
+    REL_AFIN = 64, REL_BFIN = 128, REL_WS = 256,
+};
 template <typename T>
 struct BatchEntry {
     GemmSpec<T> gemm[2];
     int32_t ngemm;
-    int32_t pad;
+    int32_t rel;
     RecompDesc<T> rd;
 };
 
@@ -71,9 +77,12 @@ hipError_t launch_readjust_ddc(hipStream_t s, const T *W, int64_t m, int64_t n, 
 template <typename T>
 hipError_t launch_fill_identity(hipStream_t s, T *I, int64_t n);
 hipError_t launch_reset_counters(hipStream_t s, BatchCounters *c);
+// Batched persistent launch: workgroup g works in slot g of `slots` (slot_bytes each) and copies
+// each update's workspace header to hdrs[HDR_WORDS * update] when the update ends.
 template <typename T>
 hipError_t launch_batched(hipStream_t s, const BatchEntry<T> *entries, const int32_t *order, int64_t n,
-                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb, int mixed);
+                          BatchCounters *ctr, int grid, int64_t lds_bytes, int pb, int mixed, char *slots,
+                          int64_t slot_bytes, int64_t *hdrs);
 template <typename T>
 hipError_t set_kernel_lds_limits(int64_t lds_bytes);
 

@@ -173,7 +173,7 @@ __device__ void ph_learned_size(const RecompDesc<T> &d, T *lds) {
         const T *rn = lds;
         const T thr_lo = rn[kk];
         float o = s_pred;
-        o = o < 0.f ? 0.f : (o > 12.f ? 12.f : o);
+        o = !(o == o) ? 12.f : (o < 0.f ? 0.f : (o > 12.f ? 12.f : o));  // NaN features: no presizing
         int pred = (int) ceilf(exp2f(o));
         if (pred > kk) pred = kk;
         int kc = pred + d.model_margin;

@@ -214,6 +214,10 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
     }
 }
 
+// Leading dimension of a Householder panel of `rows` rows staged in LDS: odd, so that the 16
+// lanes reading one row across the panel's columns hit distinct LDS banks.
+__host__ __device__ inline int panel_ld(int rows) { return rows | 1; }
+
 // ---------------------------------------------------------------------------
 // Compact-WY block reflector application with the panel held in registers.
 // Rows are dealt to waves in 64-row chunks (chunk c -> wave c % NWAVES); each
@@ -224,7 +228,8 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
 // `red` is LDS scratch of NWAVES*64 + 2*64 elements.
 // ---------------------------------------------------------------------------
 template <typename T, int PB, int RPT>
-__device__ void apply_wy(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1, bool trans, T *red) {
+__device__ void apply_wy(const T *P, int pld, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1, bool trans,
+                         T *red) {
     constexpr int NCB = WAVE / PB;  // columns per batch
     const int l = lane_id(), wv = wave_id();
     T pv[RPT][PB];
@@ -232,7 +237,7 @@ __device__ void apply_wy(const T *P, const T *Tm, int rows, T *B, int64_t ldb, i
     for (int t = 0; t < RPT; ++t) {
         const int r = l + WAVE * (wv + NWAVES * t);
 #pragma unroll
-        for (int k = 0; k < PB; ++k) pv[t][k] = (r < rows) ? P[r + k * rows] : T(0);
+        for (int k = 0; k < PB; ++k) pv[t][k] = (r < rows) ? P[r + k * pld] : T(0);
     }
     T *wfull = red + NWAVES * WAVE;
     T *tvec = wfull + WAVE;
@@ -303,34 +308,34 @@ __device__ void apply_wy(const T *P, const T *Tm, int rows, T *B, int64_t ldb, i
 // gets its own register allocation (inlining them all into one kernel makes
 // the widest variant's pressure spill the others).
 template <typename T, int PB, int RPT>
-__device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
-                                          bool trans, T *red) {
-    apply_wy<T, PB, RPT>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+__device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, int rows, T *B, int64_t ldb, int c0,
+                                             int c1, bool trans, T *red) {
+    apply_wy<T, PB, RPT>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
 }
 
 template <typename T, int PB>
-__device__ void apply_wy_dispatch(const T *P, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1, bool trans,
-                                  T *red) {
+__device__ void apply_wy_dispatch(const T *P, int pld, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
+                                  bool trans, T *red) {
     // rows <= 512 for PB 16, <= 1024 for PB 8, <= 2048 for PB 4 (pick_pb): at most 32 panel values per lane
     const int chunks = (rows + WAVE * NWAVES - 1) / (WAVE * NWAVES);
-    if (chunks <= 1) apply_wy_nl<T, PB, 1>(P, Tm, rows, B, ldb, c0, c1, trans, red);
-    else if (chunks <= 2) apply_wy_nl<T, PB, 2>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+    if (chunks <= 1) apply_wy_nl<T, PB, 1>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
+    else if (chunks <= 2) apply_wy_nl<T, PB, 2>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
     else if constexpr (PB <= 8) {
-        if (chunks <= 4) apply_wy_nl<T, PB, 4>(P, Tm, rows, B, ldb, c0, c1, trans, red);
-        else if constexpr (PB <= 4) apply_wy_nl<T, PB, 8>(P, Tm, rows, B, ldb, c0, c1, trans, red);
+        if (chunks <= 4) apply_wy_nl<T, PB, 4>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
+        else if constexpr (PB <= 4) apply_wy_nl<T, PB, 8>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
     }
 }
 
 // Build T (PB x PB, upper, column-major in LDS) for the panel P with taus tau[0..jb):
 // dlarft forward/columnwise: T[i][i] = tau_i, T[0:i, i] = -tau_i T[0:i,0:i] (V[:,0:i]^T v_i).
 template <typename T, int PB>
-__device__ void build_t(const T *P, int rows, const T *tau, int jb, T *Tm, T *gram) {
+__device__ void build_t(const T *P, int pld, int rows, const T *tau, int jb, T *Tm, T *gram) {
     const int l = lane_id();
     for (int pr = wave_id(); pr < PB * PB; pr += NWAVES) {
         const int a = pr % PB, b = pr / PB;
         if (a >= b || b >= jb) continue;
         T s = T(0);
-        for (int r = l; r < rows; r += WAVE) s += P[r + a * rows] * P[r + b * rows];
+        for (int r = l; r < rows; r += WAVE) s += P[r + a * pld] * P[r + b * pld];
         s = wave_sum(s);
         if (l == 0) gram[a + b * PB] = s;
     }
@@ -359,9 +364,17 @@ __device__ void build_t(const T *P, int rows, const T *tau, int jb, T *Tm, T *gr
     __syncthreads();
 }
 
+// Ordering key of a norm in the rank-by-comparison selections below (pivot choice, singular
+// value order): NaN, which NaN or Inf in the input produces, ranks below every number, so that
+// the positions counted from the keys always form a permutation of the indices.
+template <typename T>
+__device__ __forceinline__ T order_key(T v) {
+    return v == v ? v : T(-INFINITY);
+}
+
 // LDS elements needed by wg_geqrf / wg_apply_q for a panel of PB columns over m rows.
 __host__ __device__ inline int64_t qr_lds_elems(int64_t m, int pb) {
-    return m * pb + 2 * pb * pb + (NWAVES + 2) * WAVE + 64;
+    return (m | 1) * pb + 2 * pb * pb + (NWAVES + 2) * WAVE + 64;
 }
 
 // Window pivoting of a blocked QR at panel start j0 (column norms from `cn` when given): the jb trailing columns of largest
@@ -403,10 +416,10 @@ __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, i
     const T trail2 = wg_sum(tpart, reinterpret_cast<T *>(sel + 2 * PB) + 1);
     __syncthreads();
     for (int c = threadIdx.x; c < nt; c += NT) {
-        const T v = nrm[c];
+        const T v = order_key(nrm[c]);
         int pos = 0;
         for (int k = 0; k < nt; ++k) {
-            const T u = nrm[k];
+            const T u = order_key(nrm[k]);
             pos += (u > v) || (u == v && k < c);
         }
         if (pos < jb) sel[pos] = c;
@@ -466,7 +479,7 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
                               int64_t *hdr, int j0, const T *cn = nullptr) {
     (void) hdr;
     T *P = lds;
-    T *Tm = P + (int64_t) m * PB;
+    T *Tm = P + (int64_t) (m | 1) * PB;
     T *gram = Tm + PB * PB;
     T *red = gram + PB * PB;  // (NWAVES + 2) * WAVE
     const int nsteps = m < ncols ? m : ncols;
@@ -474,6 +487,7 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
     (void) l;
     const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
     const int rows = m - j0;
+    const int pld = panel_ld(rows);
     HCAMD_SUB_BEGIN(t_pan);
     HCAMD_SUB_BEGIN(t_piv);
     if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P, cn);
@@ -482,16 +496,16 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
 #pragma unroll
     for (int c = 0; c < PB; ++c)
         for (int r = threadIdx.x; r < rows; r += NT)
-            P[r + c * rows] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
+            P[r + c * pld] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
     __syncthreads();
     for (int jj = 0; jj < jb; ++jj) {
         T part = T(0);
         for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-            const T v = P[r + jj * rows];
+            const T v = P[r + jj * pld];
             part += v * v;
         }
         const T xnorm2 = wg_sum(part, red);
-        const T alpha = P[jj + jj * rows];
+        const T alpha = P[jj + jj * pld];
         T tau_j, beta, scal;
         if (xnorm2 == T(0)) {
             tau_j = T(0);
@@ -504,10 +518,10 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
         }
         __syncthreads();
         if (tau_j != T(0)) {
-            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) P[r + jj * rows] *= scal;
+            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) P[r + jj * pld] *= scal;
         }
         if (threadIdx.x == 0) {
-            P[jj + jj * rows] = beta;
+            P[jj + jj * pld] = beta;
             tau[j0 + jj] = tau_j;
         }
         __syncthreads();
@@ -517,10 +531,10 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
 #pragma unroll
             for (int k = 0; k < PB; ++k) acc[k] = T(0);
             for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-                const T v = P[r + jj * rows];
+                const T v = P[r + jj * pld];
 #pragma unroll
                 for (int k = 0; k < PB; ++k)
-                    if (k > jj && k < jb) acc[k] += v * P[r + k * rows];
+                    if (k > jj && k < jb) acc[k] += v * P[r + k * pld];
             }
             const T s = wave_multi_sum_fast<T, PB>(acc);
             const int idx = wave_multi_sum_index<PB>(l);
@@ -529,17 +543,17 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
             if (threadIdx.x < PB) {
                 const int k = threadIdx.x;
                 if (k > jj && k < jb) {
-                    T wsum = P[jj + k * rows];
+                    T wsum = P[jj + k * pld];
                     for (int wv = 0; wv < NWAVES; ++wv) wsum += red[wv * PB + k];
                     gram[k] = tau_j * wsum;
                 }
             }
             __syncthreads();
             for (int r = jj + threadIdx.x; r < rows; r += NT) {
-                const T v = (r == jj) ? T(1) : P[r + jj * rows];
+                const T v = (r == jj) ? T(1) : P[r + jj * pld];
 #pragma unroll
                 for (int k = 0; k < PB; ++k)
-                    if (k > jj && k < jb) P[r + k * rows] -= gram[k] * v;
+                    if (k > jj && k < jb) P[r + k * pld] -= gram[k] * v;
             }
             __syncthreads();
         }
@@ -549,15 +563,15 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
 #pragma unroll
     for (int c = 0; c < PB; ++c)
         if (c < jb)
-            for (int r = threadIdx.x; r < rows; r += NT) A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[r + c * rows];
+            for (int r = threadIdx.x; r < rows; r += NT) A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[r + c * pld];
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < PB; ++c)
         if (c < jb)
-            for (int r = threadIdx.x; r <= c && r < rows; r += NT) P[r + c * rows] = (r == c) ? T(1) : T(0);
+            for (int r = threadIdx.x; r <= c && r < rows; r += NT) P[r + c * pld] = (r == c) ? T(1) : T(0);
     __syncthreads();
     HCAMD_SUB_BEGIN(t_bt);
-    build_t<T, PB>(P, rows, tau + j0, jb, Tm, gram);
+    build_t<T, PB>(P, pld, rows, tau + j0, jb, Tm, gram);
     if (tsave) {
         for (int e = threadIdx.x; e < PB * PB; e += NT) tsave[(int64_t) (j0 / PB) * PB * PB + e] = Tm[e];
     }
@@ -570,7 +584,7 @@ template <typename T, int PB>
 __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr,
                          int64_t *hdr = nullptr) {
     T *P = lds;
-    T *Tm = P + (int64_t) m * PB;
+    T *Tm = P + (int64_t) (m | 1) * PB;
     T *red = Tm + 2 * PB * PB;
     const int nsteps = m < ncols ? m : ncols;
     if (piv) {
@@ -580,7 +594,8 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
         const int jb = wg_geqrf_panel<T, PB>(m, ncols, A, lda, tau, tsave, lds, piv, hdr, j0);
         HCAMD_SUB_BEGIN(t_up);
-        if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, m - j0, A + j0, lda, j0 + jb, ncols, true, red);
+        if (j0 + jb < ncols)
+            apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, A + j0, lda, j0 + jb, ncols, true, red);
         __syncthreads();
         HCAMD_SUB_END(t_up, hdr, 2);
     }
@@ -592,13 +607,14 @@ template <typename T, int PB>
 __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *tsave, T *B, int64_t ldb, int c0,
                            int c1, T *lds) {
     T *P = lds;
-    T *Tm = P + (int64_t) m * PB;
+    T *Tm = P + (int64_t) (m | 1) * PB;
     T *red = Tm + 2 * PB * PB;
     const int npan = (nref + PB - 1) / PB;
     for (int p = npan - 1; p >= 0; --p) {
         const int j0 = p * PB;
         const int jb = (nref - j0) < PB ? (nref - j0) : PB;
         const int rows = m - j0;
+        const int pld = panel_ld(rows);
 #pragma unroll
         for (int c = 0; c < PB; ++c)
             for (int r = threadIdx.x; r < rows; r += NT) {
@@ -607,11 +623,11 @@ __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *ts
                     if (r == c) v = T(1);
                     else if (r > c) v = A[(j0 + r) + (int64_t) (j0 + c) * lda];
                 }
-                P[r + c * rows] = v;
+                P[r + c * pld] = v;
             }
         for (int e = threadIdx.x; e < PB * PB; e += NT) Tm[e] = tsave[(int64_t) p * PB * PB + e];
         __syncthreads();
-        apply_wy_dispatch<T, PB>(P, Tm, rows, B + j0, ldb, c0, c1, false, red);
+        apply_wy_dispatch<T, PB>(P, pld, Tm, rows, B + j0, ldb, c0, c1, false, red);
         __syncthreads();
     }
 }
@@ -652,7 +668,7 @@ __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
     const int m = d.n, ncols = z.mu;
     const int nsteps = m < ncols ? m : ncols;
     T *P = lds;
-    T *Tm = P + (int64_t) m * PB;
+    T *Tm = P + (int64_t) (m | 1) * PB;
     T *red = Tm + 2 * PB * PB;
     for (int c = threadIdx.x; c < ncols; c += NT) w.piv[c] = c;
     __syncthreads();
@@ -671,7 +687,8 @@ __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
             return;
         }
         wg_geqrf_panel<T, PB>(m, ncols, w.y, m, w.tau_y, w.t_y, lds, nullptr, w.hdr, j0);
-        if (j0 + jb < ncols) apply_wy_dispatch<T, PB>(P, Tm, m - j0, w.y + j0, m, j0 + jb, ncols, true, red);
+        if (j0 + jb < ncols)
+            apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, w.y + j0, m, j0 + jb, ncols, true, red);
         __syncthreads();
     }
 }
@@ -883,10 +900,10 @@ __device__ void ph_svd_finish(const RecompDesc<T> &d, int sweeps) {
     }
     __syncthreads();
     for (int j = threadIdx.x; j < kt; j += NT) {
-        const T sj = w.sig[j];
+        const T sj = order_key(w.sig[j]);
         int pos = 0;
         for (int k = 0; k < kt; ++k) {
-            const T sk = w.sig[k];
+            const T sk = order_key(w.sig[k]);
             pos += (sk > sj) || (sk == sj && k < j);
         }
         w.perm[pos] = j;
@@ -948,6 +965,148 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
 #endif
     }
     ph_svd_finish<T>(d, sweeps);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: the core Jacobi without V.  The rotations only orthogonalise the columns of the
+// core X (X V1 = Z); the right factor is recovered afterwards from the left one: X = Z V1^T
+// gives X^T U = V1 Sigma for U = Z Sigma^-1 (ph_recon_nov).  Dropping V halves the Jacobi's data
+// and frees the registers that let the tiles double their width (half the passes over X per
+// sweep).  A core of at most BATCH_LDS_BYTES / sizeof(T) - NOV_GBUF elements is moved into the
+// kernel's dynamic LDS for the whole iteration, so its sweeps read no global memory.
+// ---------------------------------------------------------------------------
+constexpr int64_t BATCH_LDS_BYTES = 78 * 1024;  // dynamic LDS per workgroup of the batched kernel
+constexpr int NOV_GBUF = 16;                     // reduction slots ahead of the LDS-resident core
+
+template <typename T>
+__host__ __device__ constexpr int64_t nov_lds_cap() {
+    return BATCH_LDS_BYTES / (int64_t) sizeof(T) - NOV_GBUF;
+}
+
+// LDS-resident core: J (rows x ncols, ld rows) at element NOV_GBUF of the dynamic LDS, the
+// reduction slots before it.  The pointers are formed here from the dynamic LDS symbol so that
+// every access of the iteration is an LDS instruction.
+template <typename T, int RPL, int TW>
+__device__ HCAMD_PHASE_ATTR int jacobi_nov_lds_nl(int rows, int ncols) {
+    extern __shared__ unsigned char smem_raw[];
+    T *gbuf = reinterpret_cast<T *>(smem_raw);
+    return wg_block_jacobi<T, RPL, 0, TW>(gbuf + NOV_GBUF, rows, ncols, rows, nullptr, 0, JACOBI_MAX_SWEEPS, gbuf);
+}
+
+template <typename T, int RPL, int TW>
+__device__ HCAMD_PHASE_ATTR int jacobi_nov_nl(T *J, int rows, int ncols, T *gbuf) {
+    return wg_block_jacobi<T, RPL, 0, TW>(J, rows, ncols, rows, nullptr, 0, JACOBI_MAX_SWEEPS, gbuf);
+}
+
+template <typename T, int RPL, int TW>
+__device__ int jacobi_nov_shape(T *J, int rows, int ncols, T *lds, bool resident) {
+    return resident ? jacobi_nov_lds_nl<T, RPL, TW>(rows, ncols) : jacobi_nov_nl<T, RPL, TW>(J, rows, ncols, lds);
+}
+
+// One-sided Jacobi of J (rows x ncols, ld rows) in place, without V; `lds` is the kernel's
+// dynamic LDS (BATCH_LDS_BYTES).  Returns the sweeps used.
+template <typename T>
+__device__ int run_jacobi_nov(T *J, int rows, int ncols, T *lds) {
+    const int64_t ne = (int64_t) rows * ncols;
+    const bool resident = ne <= nov_lds_cap<T>();
+    T *Jl = lds + NOV_GBUF;
+    if (resident) {
+        for (int64_t e = threadIdx.x; e < ne; e += NT) Jl[e] = J[e];
+        __syncthreads();
+    }
+    int sweeps;
+    if (rows <= 64) sweeps = jacobi_nov_shape<T, 1, 16>(J, rows, ncols, lds, resident);
+    else if (rows <= 128) sweeps = jacobi_nov_shape<T, 2, 16>(J, rows, ncols, lds, resident);
+    else if (rows <= 192) sweeps = jacobi_nov_shape<T, 3, 8>(J, rows, ncols, lds, resident);
+    else if (rows <= 256) sweeps = jacobi_nov_shape<T, 4, 8>(J, rows, ncols, lds, resident);
+    else if (rows <= 384) sweeps = jacobi_nov_shape<T, 6, 8>(J, rows, ncols, lds, resident);
+    else if (rows <= 512) sweeps = jacobi_nov_shape<T, 8, 4>(J, rows, ncols, lds, resident);
+    else sweeps = jacobi_nov_shape<T, 12, 4>(J, rows, ncols, lds, resident);
+    if (resident) {
+        __syncthreads();
+        for (int64_t e = threadIdx.x; e < ne; e += NT) J[e] = Jl[e];
+        __syncthreads();
+    }
+    return sweeps;
+}
+
+// Core SVD of the batched form: the Jacobi without V, then sigma, order and rank (ph_svd_finish).
+template <typename T>
+__device__ void ph_svd_nov(const RecompDesc<T> &d, T *lds) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    int sweeps = 0;
+    if (z.mu > JACOBI_ROWS_MAX) {
+        if (threadIdx.x == 0) w.hdr[HDR_STATUS] |= STATUS_TOO_BIG;
+    } else {
+        HCAMD_SUB_BEGIN(t_jac);
+        if (w.hdr[HDR_CQR]) sweeps = run_jacobi_nov<T>(w.j2, z.kt, z.kt, lds);
+        else sweeps = run_jacobi_nov<T>(w.j, z.mu, z.kt, lds);
+        HCAMD_SUB_END(t_jac, w.hdr, 3);
+#ifdef HCAMD_PHASE_TIMING
+        if (threadIdx.x == 0) w.hdr[HDR_SUB + 7] = sweeps + 1000 * z.kt;  // timing build: sweeps and core width
+#endif
+    }
+    ph_svd_finish<T>(d, sweeps);
+}
+
+// Reconstruction after ph_svd_nov, all r columns of the new rank:
+//   C_U  = Q_U [Q2] [Z_perm Sigma^-1; 0]       (the left factor, as ph_recon side 0)
+//   vtmp = Q_Y [X^T C_U(0:xrows, :); 0]        (X^T U = V1 Sigma: the right factor carrying Sigma)
+// X is the core the Jacobi orthogonalised, still intact in the workspace: R2, the upper triangle
+// of J after the core QR, or P R_Y[0:kt, :]^T, read from Y through the pivoting.
+template <typename T, int PB>
+__device__ void ph_recon_nov(const RecompDesc<T> &d, T *lds) {
+    Ws<T> w = ws_of(d);
+    const Dims z = dims_of(d, w);
+    const int r = (int) w.hdr[HDR_RANK];
+    if (r < 1) return;
+    const int m = d.m, n = d.n, mu = z.mu, kk = z.kk, kt = z.kt;
+    const bool cqr = w.hdr[HDR_CQR] != 0;
+    const T *zc = cqr ? w.j2 : w.j;
+    const int xrows = cqr ? kt : mu;
+    T *U = d.cdata;  // ld m
+    for (int64_t e = threadIdx.x; e < (int64_t) m * r; e += NT) {
+        const int i = (int) (e % m), c = (int) (e / m);
+        T v = T(0);
+        if (i < xrows && c < kt) {
+            const int j = w.perm[c];
+            const T sg = w.sig[j];
+            v = (sg > T(0)) ? zc[i + (int64_t) j * xrows] / sg : T(0);
+        }
+        U[i + (int64_t) c * m] = v;
+    }
+    for (int64_t e = threadIdx.x; e < (int64_t) (n - kt) * r; e += NT) {
+        const int i = kt + (int) (e % (n - kt)), c = (int) (e / (n - kt));
+        w.vtmp[i + (int64_t) c * n] = T(0);
+    }
+    __syncthreads();
+    HCAMD_SUB_BEGIN(t_rf);
+    const int tm = (kt + GT_M - 1) / GT_M, tn = (r + GT_N - 1) / GT_N;
+    if (cqr) {
+        const T *R2 = w.j;  // R2[i, j] = J[i + j mu] for i <= j
+        auto fa = [=](int j, int i) -> T { return i <= j ? R2[i + (int64_t) j * mu] : T(0); };
+        auto fb = [=](int i, int c) -> T { return U[i + (int64_t) c * m]; };
+        for (int t = 0; t < tm * tn; ++t) wg_gemm_tile_fn<T>(kt, r, kt, fa, fb, w.vtmp, n, t % tm, t / tm, lds);
+    } else {
+        const T *Y = w.y;  // X[piv[i], j] = R_Y[j, i] = Y[j + i n] for i >= j
+        const int32_t *piv = w.piv;
+        auto fa = [=](int j, int i) -> T { return i >= j ? Y[j + (int64_t) i * n] : T(0); };
+        auto fb = [=](int i, int c) -> T { return U[piv[i] + (int64_t) c * m]; };
+        for (int t = 0; t < tm * tn; ++t) wg_gemm_tile_fn<T>(kt, r, mu, fa, fb, w.vtmp, n, t % tm, t / tm, lds);
+    }
+    __syncthreads();
+    HCAMD_SUB_END(t_rf, w.hdr, 6);
+    if (cqr) {
+        wg_apply_q<T, PB>(mu, kt, w.j, mu, w.t_j, U, m, 0, r, lds);
+        __syncthreads();
+    }
+    HCAMD_SUB_BEGIN(t_aq);
+    wg_apply_q<T, PB>(m, mu, w.us, m, w.t_u, U, m, 0, r, lds);
+    __syncthreads();
+    // [X^T U; 0] vanishes below row kt, where reflectors kt.. of Q_Y act as the identity
+    wg_apply_q<T, PB>(n, kt < kk ? kt : kk, w.y, n, w.t_y, w.vtmp, n, 0, r, lds);
+    HCAMD_SUB_END(t_aq, w.hdr, 4);
 }
 
 // ---------------------------------------------------------------------------

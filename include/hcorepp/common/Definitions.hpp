@@ -1,6 +1,6 @@
 // Drop-in for include/hcorepp/common/Definitions.hpp of ecrc/hcorepp: the enums the
-// TLR GEMM surface uses.  blas::Op / blas::Layout are defined here unless BLAS++'s
-// blas.hh was included first (same enumerator values: 'N','T','C' and 'C','R').
+// TLR GEMM and Cholesky-flow surface uses.  blas::Op / Layout / Uplo / Side / Diag are
+// defined here unless BLAS++'s blas.hh was included first (same enumerator values).
 #pragma once
 
 #include <cstddef>
@@ -10,6 +10,9 @@
 namespace blas {
 enum class Op : char { NoTrans = 'N', Trans = 'T', ConjTrans = 'C' };
 enum class Layout : char { ColMajor = 'C', RowMajor = 'R' };
+enum class Uplo : char { Upper = 'U', Lower = 'L', General = 'G' };
+enum class Side : char { Left = 'L', Right = 'R' };
+enum class Diag : char { NonUnit = 'N', Unit = 'U' };
 }  // namespace blas
 #endif
 

@@ -125,9 +125,11 @@ hcorepp_status hcorepp_ctx_event_elapsed(hcorepp_ctx *ctx, int slot0, int slot1,
  *  - All 8 dense/compressed combinations.  Compressed C requires opA = opB = 'N' (the
  *    reference's CompressedTile::Gemm ignores the ops, Compressed.cpp:331-380); other ops
  *    return HCOREPP_ERR_UNSUPPORTED.  Dense C supports 'N' and 'T'.
- *  - cholesky != 0 (the reference's aCholesky operand re-plumbing, HCore.cpp:160-174, used
- *    only by its TLR Cholesky driver) returns HCOREPP_ERR_UNSUPPORTED: the TLR Cholesky here
- *    passes the transposed panel tile explicitly (hcorepp_transpose_batch_d below).
+ *  - cholesky != 0: the Cholesky-flow Gemm (the reference's aCholesky, HCore.cpp:160-174):
+ *    compressed A, B, C in that flow's convention (V stored cols x rank), opA 'N', opB 'T',
+ *    C <- beta C + alpha A B^T; other combinations return HCOREPP_ERR_UNSUPPORTED.  (The
+ *    batched TLR Cholesky driver instead passes the transposed panel tile explicitly,
+ *    hcorepp_transpose_batch_d below, and stays in the [U | V r x n] convention.)
  *  - Plain form: the recompression runs as a chain of launches on the context stream and the
  *    new rank is read back with one stream synchronisation, as the reference does
  *    (src/kernels/cuda/CudaKernels.cu:656-697).  C->rank is updated on return.
@@ -158,9 +160,15 @@ hcorepp_status hcorepp_compress_d(hcorepp_ctx *ctx, int64_t rows, int64_t cols, 
 hcorepp_status hcorepp_compress_s(hcorepp_ctx *ctx, int64_t rows, int64_t cols, const float *A, int64_t lda,
                                   const hcorepp_cparams *params, hcorepp_tile *out);
 
-/* Workspace (elements of T) one update needs: reference HCore<T>::CalculateMemoryPoolSize
- * (src/api/HCore.cpp:417-480).  The engine's layout differs, so the number is this engine's. */
+/* Workspace (elements of T) one update C = A B + C (opA = opB = 'N') takes from the context arena:
+ * reference HCore<T>::CalculateMemoryPoolSize (src/api/HCore.cpp:417-480), which the reference's
+ * stream drivers use to presize one pool per stream (examples/matrix_multiplication/
+ * par_fixed_rank_streams_main.cpp:86-122).  The engine's layout differs, so the number is this
+ * engine's; a context created with arena_bytes = this * sizeof(T) never grows its arena for the
+ * update. */
 size_t hcorepp_gemm_pool_size_d(const hcorepp_tile *A, const hcorepp_tile *B, const hcorepp_tile *C,
+                                const hcorepp_cparams *params);
+size_t hcorepp_gemm_pool_size_s(const hcorepp_tile *A, const hcorepp_tile *B, const hcorepp_tile *C,
                                 const hcorepp_cparams *params);
 
 /* ---------------------------------------------------------------------------
@@ -234,6 +242,37 @@ hcorepp_status hcorepp_syrk_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_t
  * counterpart in the reference, whose aCholesky Gemm branch (HCore.cpp:160-174) re-plumbs the
  * operands instead. */
 hcorepp_status hcorepp_transpose_batch_d(hcorepp_ctx *ctx, int64_t n, const hcorepp_tile *src, hcorepp_tile *dst);
+
+/* ---------------------------------------------------------------------------
+ * Cholesky-flow tile operations with the reference's argument meaning and its tile convention
+ * for that flow: a compressed tile is U V^T with V stored cols x rank (ld cols) in the V slot.
+ * The Gemm of that flow is hcorepp_gemm_d / _s with cholesky = 1 (compressed A, B, C; opA 'N',
+ * opB 'T': C <- beta C + alpha A B^T), replacing HCore<T>::Gemm(..., aCholesky = true)
+ * (reference src/api/HCore.cpp:160-174, src/operators/concrete/Compressed.cpp:273-276, 458-476,
+ * 529-595, 669-673).
+ * Replaces HCore<T>::Potrf (reference include/hcorepp/api/HCore.hpp:78-90, src/api/HCore.cpp:586-622):
+ * LAPACK potrf of a dense square tile, uplo 'L' (A = L L^T) or 'U' (A = U^T U); the other strict
+ * triangle is left untouched.  A non-positive leading minor returns HCOREPP_ERR_NUMERIC (the
+ * reference ignores LAPACK's info).  Synchronises the context stream. */
+hcorepp_status hcorepp_tile_potrf_d(hcorepp_ctx *ctx, char uplo, hcorepp_tile *A);
+hcorepp_status hcorepp_tile_potrf_s(hcorepp_ctx *ctx, char uplo, hcorepp_tile *A);
+/* Replaces HCore<T>::Trsm (reference include/hcorepp/api/HCore.hpp:92-106, src/api/HCore.cpp:624-647):
+ * BLAS trsm(side 'L'/'R', uplo 'L'/'U', trans 'N'/'T'/'C', diag 'N'/'U', alpha) with the dense
+ * square A on the V factor of the compressed B, viewed as B.rows x B.rank with B's leading
+ * dimension: V <- alpha op(A)^{-1} V (left) or alpha V op(A)^{-1} (right). */
+hcorepp_status hcorepp_tile_trsm_d(hcorepp_ctx *ctx, char side, char uplo, char trans, char diag, double alpha,
+                                   const hcorepp_tile *A, hcorepp_tile *B);
+hcorepp_status hcorepp_tile_trsm_s(hcorepp_ctx *ctx, char side, char uplo, char trans, char diag, float alpha,
+                                   const hcorepp_tile *A, hcorepp_tile *B);
+/* Replaces HCore<T>::Syrk (reference include/hcorepp/api/HCore.hpp:66-77, src/api/HCore.cpp:484-584)
+ * into a dense C.  Dense A: BLAS syrk on C's uplo ('L'/'U') triangle, C <- alpha op(A) op(A)^T +
+ * beta C with n = C.rows and k = C.cols, as the reference passes them.  Compressed A = U V^T:
+ * C <- beta C - U (V^T V) U^T (alpha is not used, as in the reference), then the strict upper
+ * (uplo 'L') or lower ('U') triangle set to zero; uplo 'G' keeps both. */
+hcorepp_status hcorepp_tile_syrk_d(hcorepp_ctx *ctx, double alpha, const hcorepp_tile *A, char opA, char uplo,
+                                   double beta, hcorepp_tile *C);
+hcorepp_status hcorepp_tile_syrk_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile *A, char opA, char uplo,
+                                   float beta, hcorepp_tile *C);
 
 /* ---------------------------------------------------------------------------
  * Learned rank predictor (no counterpart in the reference).  Per-update features are computed on

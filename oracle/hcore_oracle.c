@@ -630,3 +630,30 @@ double orc_batch_gemm(int64_t n, double alpha, const orc_tile *A, const orc_tile
     clock_gettime(CLOCK_MONOTONIC, &t1);
     return (double) (t1.tv_sec - t0.tv_sec) + 1e-9 * (double) (t1.tv_nsec - t0.tv_nsec);
 }
+
+/* CPU baseline driver of a tile GEMM C(i, j) += sum_k A(i, k) B(k, j) over a grid of C tiles:
+ * OpenMP over the C tiles (collapse of the two tile loops), each tile's updates in k order, as
+ * the reference's OpenMP example does (examples/matrix_multiplication/omp_main.cpp:85-126, which
+ * sets MKL to ceil(threads / #C tiles) threads).  A is ti x tk tiles, B tk x tj, C ti x tj, all
+ * row-major arrays of tile descriptors; omp_threads OpenMP threads, lapack_threads MKL threads
+ * inside each.  Returns seconds. */
+double orc_tlr_gemm_omp(int64_t ti, int64_t tj, int64_t tk, double alpha, const orc_tile *A, const orc_tile *B,
+                        double beta, orc_tile *C, const orc_params *p, int omp_threads, int lapack_threads) {
+    need_lapack();
+    if (omp_threads < 1) omp_threads = 1;
+    if (lapack_threads < 1) lapack_threads = 1;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+#pragma omp parallel num_threads(omp_threads)
+    {
+        orc_set_lapack_threads(lapack_threads);
+#pragma omp for collapse(2) schedule(dynamic, 1)
+        for (int64_t i = 0; i < ti; ++i)
+            for (int64_t j = 0; j < tj; ++j)
+                for (int64_t k = 0; k < tk; ++k)
+                    orc_hcore_gemm(alpha, &A[i * tk + k], 'N', &B[k * tj + j], 'N', beta, &C[i * tj + j], p, NULL,
+                                   NULL);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    return (double) (t1.tv_sec - t0.tv_sec) + 1e-9 * (double) (t1.tv_nsec - t0.tv_nsec);
+}

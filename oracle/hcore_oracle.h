@@ -62,6 +62,8 @@ int orc_dpotrf(char uplo, int64_t n, double *a, int64_t lda);
  * updates (MKL sequential per thread).  Returns wall seconds. */
 double orc_batch_gemm(int64_t n, double alpha, const orc_tile *A, const orc_tile *B, double beta, orc_tile *C,
                       const orc_params *p, int threads, int64_t *ranks_out);
+double orc_tlr_gemm_omp(int64_t ti, int64_t tj, int64_t tk, double alpha, const orc_tile *A, const orc_tile *B,
+                        double beta, orc_tile *C, const orc_params *p, int omp_threads, int lapack_threads);
 
 #ifdef __cplusplus
 }

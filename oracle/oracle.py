@@ -80,6 +80,10 @@ def lib():
         L.orc_batch_gemm.restype = C.c_double
         L.orc_batch_gemm.argtypes = [C.c_int64, C.c_double, C.POINTER(_Tile), C.POINTER(_Tile), C.c_double,
                                      C.POINTER(_Tile), C.POINTER(_Params), C.c_int, C.c_void_p]
+        L.orc_tlr_gemm_omp.restype = C.c_double
+        L.orc_tlr_gemm_omp.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double, C.POINTER(_Tile),
+                                       C.POINTER(_Tile), C.c_double, C.POINTER(_Tile), C.POINTER(_Params), C.c_int,
+                                       C.c_int]
         _lib = L
     return _lib
 
@@ -235,3 +239,19 @@ def batch_gemm(alpha, As, Bs, beta, Cs, params: Params, threads: int):
     for i in range(n):
         Cs[i].update_from(TC[i])
     return sec, ranks
+
+
+def tlr_gemm_omp(alpha, A, B, beta, Cg, params: Params, omp_threads: int, lapack_threads: int):
+    """Tile GEMM C(i, j) += sum_k A(i, k) B(k, j) with OpenMP over the C tiles and `lapack_threads`
+    MKL threads per tile (the reference's OpenMP example, omp_main.cpp:85-126).  A, B, Cg are lists
+    of lists of tiles (ti x tk, tk x tj, ti x tj); the C tiles are updated.  Returns seconds."""
+    ti, tk, tj = len(A), len(A[0]), len(B[0])
+    TA = (_Tile * (ti * tk))(*[A[i][k].c() for i in range(ti) for k in range(tk)])
+    TB = (_Tile * (tk * tj))(*[B[k][j].c() for k in range(tk) for j in range(tj)])
+    TC = (_Tile * (ti * tj))(*[Cg[i][j].c() for i in range(ti) for j in range(tj)])
+    sec = lib().orc_tlr_gemm_omp(ti, tj, tk, float(alpha), TA, TB, float(beta), TC, C.byref(params.c()),
+                                 int(omp_threads), int(lapack_threads))
+    for i in range(ti):
+        for j in range(tj):
+            Cg[i][j].update_from(TC[i * tj + j])
+    return sec

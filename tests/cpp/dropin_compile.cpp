@@ -14,7 +14,81 @@ using namespace hcorepp::operators;
 using namespace hcorepp::kernels;
 using namespace hcorepp::dataunits;
 
+// "chol <in> <out>": the right-looking TLR Cholesky of a T x T tile matrix through the reference's
+// Cholesky-flow API (HCore<double>::Potrf / Trsm / Syrk and Gemm with aCholesky = true; compressed
+// tiles U V^T with V stored nb x rank).  File format (int64 / double, column-major): T, nb, acc;
+// the T dense diagonal tiles; then for i > k in row order: rank, max_rank, U (nb x rank), V (nb x rank).
+// The output file holds the factor in the same format (final ranks).
+static int run_cholesky(const char *in_path, const char *out_path) {
+    FILE *f = std::fopen(in_path, "rb");
+    if (!f) return 2;
+    int64_t T = 0, nb = 0;
+    double acc = 0;
+    bool ok = std::fread(&T, 8, 1, f) == 1 && std::fread(&nb, 8, 1, f) == 1 && std::fread(&acc, 8, 1, f) == 1;
+    RunContext ctx;
+    std::vector<DenseTile<double> *> D((size_t) T, nullptr);
+    std::vector<CompressedTile<double> *> L((size_t) (T * T), nullptr);
+    std::vector<double> buf((size_t) (2 * nb * nb));
+    for (int64_t k = 0; ok && k < T; ++k) {
+        ok = std::fread(buf.data(), 8, (size_t) (nb * nb), f) == (size_t) (nb * nb);
+        D[(size_t) k] = new DenseTile<double>((size_t) nb, (size_t) nb, buf.data(), (size_t) nb, ctx);
+    }
+    for (int64_t i = 0; ok && i < T; ++i)
+        for (int64_t k = 0; ok && k < i; ++k) {
+            int64_t r = 0, mr = 0;
+            ok = std::fread(&r, 8, 1, f) == 1 && std::fread(&mr, 8, 1, f) == 1 &&
+                 std::fread(buf.data(), 8, (size_t) (2 * nb * r), f) == (size_t) (2 * nb * r);
+            L[(size_t) (i * T + k)] = new CompressedTile<double>((size_t) nb, (size_t) nb, buf.data(), (size_t) nb,
+                                                                 (size_t) r, (size_t) mr, ctx);
+        }
+    std::fclose(f);
+    if (!ok) return 3;
+    size_t flops = 0;
+    MemoryUnit<double> &unit = MemoryHandler<double>::GetInstance().GetMemoryUnit();
+    const CompressionParameters params(acc);
+    for (int64_t k = 0; k < T; ++k) {
+        HCore<double>::Potrf(*D[(size_t) k], blas::Uplo::Lower, ctx, flops, unit);
+        for (int64_t i = k + 1; i < T; ++i)
+            HCore<double>::Trsm(blas::Side::Left, blas::Uplo::Lower, blas::Op::NoTrans, blas::Diag::NonUnit, 1.0,
+                                *D[(size_t) k], *L[(size_t) (i * T + k)], ctx, flops, unit);
+        for (int64_t i = k + 1; i < T; ++i)
+            HCore<double>::Syrk(-1.0, *L[(size_t) (i * T + k)], blas::Op::NoTrans, blas::Uplo::Lower, 1.0,
+                                *D[(size_t) i], ctx, flops, unit);
+        for (int64_t j = k + 1; j < T; ++j)
+            for (int64_t i = j + 1; i < T; ++i)
+                HCore<double>::Gemm(-1.0, *L[(size_t) (i * T + k)], blas::Op::NoTrans, *L[(size_t) (j * T + k)],
+                                    blas::Op::Trans, 1.0, *L[(size_t) (i * T + j)], ctx, flops, unit, params, true);
+    }
+    ctx.Sync();
+    FILE *o = std::fopen(out_path, "wb");
+    if (!o) return 4;
+    std::fwrite(&T, 8, 1, o);
+    std::fwrite(&nb, 8, 1, o);
+    std::fwrite(&acc, 8, 1, o);
+    for (int64_t k = 0; k < T; ++k) {
+        ThrowIfError(hcorepp_memcpy(ctx.Handle(), buf.data(), D[(size_t) k]->GetData(), nb * nb * sizeof(double)));
+        std::fwrite(buf.data(), 8, (size_t) (nb * nb), o);
+    }
+    for (int64_t i = 0; i < T; ++i)
+        for (int64_t k = 0; k < i; ++k) {
+            CompressedTile<double> *t = L[(size_t) (i * T + k)];
+            const int64_t r = (int64_t) t->GetTileRank(), mr = (int64_t) t->GetMaxRank();
+            ThrowIfError(hcorepp_memcpy(ctx.Handle(), buf.data(), t->GetUMatrix(), nb * r * sizeof(double)));
+            ThrowIfError(hcorepp_memcpy(ctx.Handle(), buf.data() + nb * r, t->GetVMatrix(), nb * r * sizeof(double)));
+            std::fwrite(&r, 8, 1, o);
+            std::fwrite(&mr, 8, 1, o);
+            std::fwrite(buf.data(), 8, (size_t) (2 * nb * r), o);
+        }
+    std::fclose(o);
+    for (auto *d : D) delete d;
+    for (auto *t : L) delete t;
+    std::printf("dropin_compile: cholesky %lld x %lld tiles nb=%lld, flops %zu\n", (long long) T, (long long) T,
+                (long long) nb, flops);
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc == 4 && std::strcmp(argv[1], "chol") == 0) return run_cholesky(argv[2], argv[3]);
     if (argc < 2 || std::strcmp(argv[1], "run") != 0) {
         std::printf("dropin_compile: built (%s)\n", hcorepp_version());
         return 0;

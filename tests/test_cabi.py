@@ -113,3 +113,61 @@ def test_argument_checks_fail_before_any_device_work():
     out = ctypes.c_void_p()
     assert L.hcorepp_batch_create(None, 1, ctypes.byref(t), ctypes.byref(t), ctypes.byref(t), 1.0, 1.0,
                                   ctypes.byref(p), 0, ctypes.byref(out)) == -1
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_tlr_cholesky_matches_oracle(tmp_path):
+    """A 3 x 3-tile TLR Cholesky (nb = 128, exp(-r/0.1) + 1e-2 I over points in the unit cube, acc
+    1e-8) run by a C++ caller through the reference's Cholesky-flow API (HCore<double>::Potrf,
+    Trsm, Syrk and Gemm with aCholesky = true, tiles U V^T with V stored nb x rank), against the
+    oracle's tile sequence (oracle/cholesky.py)."""
+    import subprocess
+    import numpy as np
+    from hcorepp_amd.workloads import TLRMatrix
+    from oracle import cholesky as OC
+    exe = build_dropin(tmp_path)
+    nb, T, acc = 128, 3, 1e-8
+    M = TLRMatrix(T * nb, nb, 3, 0.1, acc, 20261207, nugget=1e-2)
+    inp, out = str(tmp_path / "chol_in.bin"), str(tmp_path / "chol_out.bin")
+    with open(inp, "wb") as f:
+        np.array([T, nb], dtype=np.int64).tofile(f)
+        np.array([acc], dtype=np.float64).tofile(f)
+        for k in range(T):
+            np.asfortranarray(M.tile(k, k)[1]).reshape(-1, order="F").tofile(f)
+        for i in range(T):
+            for k in range(i):
+                _, u, v, mr = M.tile(i, k)
+                np.array([u.shape[1], mr], dtype=np.int64).tofile(f)
+                u.reshape(-1, order="F").tofile(f)
+                np.asfortranarray(v.T).reshape(-1, order="F").tofile(f)  # V stored nb x rank
+    res = subprocess.run([exe, "chol", inp, out], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = open(out, "rb").read()
+    pos = 24
+    Ld = np.zeros((T * nb, T * nb))
+    ranks = {}
+    for k in range(T):
+        d = np.frombuffer(raw, dtype=np.float64, count=nb * nb, offset=pos).reshape(nb, nb, order="F")
+        Ld[k * nb:(k + 1) * nb, k * nb:(k + 1) * nb] = np.tril(d)
+        pos += 8 * nb * nb
+    for i in range(T):
+        for k in range(i):
+            r, _ = np.frombuffer(raw, dtype=np.int64, count=2, offset=pos)
+            pos += 16
+            u = np.frombuffer(raw, dtype=np.float64, count=nb * r, offset=pos).reshape(nb, r, order="F")
+            v = np.frombuffer(raw, dtype=np.float64, count=nb * r, offset=pos + 8 * nb * r).reshape(nb, r, order="F")
+            pos += 16 * nb * r
+            Ld[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb] = u @ v.T
+            ranks[(i, k)] = int(r)
+    diag = [M.tile(k, k)[1].copy(order="F") for k in range(T)]
+    off = {(i, k): M.tile(i, k)[1:] for i in range(T) for k in range(i)}
+    Fo = OC.TLRCholesky(diag, off, acc).factor()
+    Lo = Fo.dense_factor()
+    ro = Fo.ranks()
+    diff = sum(abs(ranks[k] - ro[k]) for k in ro)
+    assert diff <= 1, (ranks, ro)
+    assert np.linalg.norm(Ld - Lo) / np.linalg.norm(Lo) < (1e-11 if diff == 0 else 10 * acc)
+    A = M.dense_block_range(0, M.n)
+    be = np.linalg.norm(A - Ld @ Ld.T) / np.linalg.norm(A)
+    be_o = np.linalg.norm(A - Lo @ Lo.T) / np.linalg.norm(A)
+    assert be <= 1.5 * be_o + 1e-14, (be, be_o)

@@ -1,0 +1,152 @@
+"""GPU parity of the benchmark's workloads at their real tile sizes, against the CPU oracle:
+
+* the 4 x 4-tile, nb = 256 TLR GEMM C += A B of the exponential covariance exp(-r/0.1) over 1024
+  points in the unit square (seed 20261202), acc 1e-8, A = B = C: the tile loop of the
+  reference's OpenMP example (examples/matrix_multiplication/omp_main.cpp:112-126), 64 updates
+  (24 CCC, 12 DCC, 12 CDC, 12 CCD, 4 DDD) through the plain HIP path, each compared with the oracle
+  run on the same input (the device's C before the update);
+* the TLR Cholesky of an 8 x 8-tile, nb = 512 matrix exp(-r/0.1) + 1e-2 I over 4096 points in the
+  unit cube, acc 1e-8, against the oracle's tile sequence (oracle/cholesky.py);
+* an 8192-update, nb = 512 batch of the benchmark's distribution with learned core sizing: every
+  rank against the oracle's, results on a sample within the accuracy of the oracle's.
+
+Tolerances: ranks equal unless an oracle singular value lies within 1e3 eps sigma_0 of the
+threshold; products of compressed results within 1e-10 of the oracle's (relative, when the ranks
+agree); dense results 1e-13.
+"""
+import numpy as np
+import pytest
+
+from helpers import EPS, rank_tie, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    import hcorepp_amd as H
+    from oracle import oracle as O
+    return H, O
+
+
+def test_tlr_gemm_4x4_nb256_matches_oracle(ctx):
+    H, O = _mods()
+    from hcorepp_amd.workloads import TLRMatrix
+    acc = 1e-8
+    M = TLRMatrix(1024, 256, 2, 0.1, acc, 20261202)
+    T = M.T
+    params, oparams = H.CompressionParameters(acc), O.Params(acc)
+
+    def dev(t):
+        return H.DenseTile(t[1]) if t[0] == "D" else H.CompressedTile(t[1], t[2], max_rank=t[3])
+
+    def orc(t):
+        return O.Tile.dense(t[1].copy(order="F")) if t[0] == "D" else O.Tile.compressed(t[1], t[2], max_rank=t[3])
+
+    def dense(t):
+        return t[1] if t[0] == "D" else t[1] @ t[2]
+
+    tiles = {(i, k): dev(M.tile(i, k)) for i in range(T) for k in range(T)}  # A and B (read only)
+    kinds, ties, ranks = {}, 0, []
+    final_err = []
+    for i in range(T):
+        for j in range(T):
+            tc = M.tile(i, j)
+            C = dev(tc)
+            ref = dense(tc).copy()
+            for k in range(T):
+                ta, tb = M.tile(i, k), M.tile(k, j)
+                kind = ta[0] + tb[0] + tc[0]
+                kinds[kind] = kinds.get(kind, 0) + 1
+                print(f"[case] tile ({i},{j}) k={k} {kind}", flush=True)
+                if tc[0] == "D":
+                    oC = O.Tile.dense(C.to_numpy())
+                else:
+                    u, v = C.factors()
+                    oC = O.Tile.compressed(u, v, max_rank=tc[3])
+                _, so = O.hcore_gemm(1.0, orc(ta), orc(tb), 1.0, oC, oparams, want_sigma=True)
+                H.HCore.gemm(1.0, tiles[(i, k)], tiles[(k, j)], 1.0, C, ctx, params)
+                got, want = C.to_dense(), oC.to_dense()
+                if tc[0] == "D":
+                    assert rel_fro(got, want) < 1e-13, (i, j, k)
+                else:
+                    assert rank_tie(so, oC.rank, C.rank, acc), (i, j, k, oC.rank, C.rank)
+                    ties += C.rank != oC.rank
+                    if C.rank == oC.rank:
+                        assert rel_fro(got, want) < 1e-10, (i, j, k, rel_fro(got, want))
+                    ranks.append(C.rank)
+                ref += dense(ta) @ dense(tb)
+            final_err.append(rel_fro(C.to_dense(), ref))
+    assert kinds == {"CCC": 24, "DCC": 12, "CDC": 12, "CCD": 12, "DDD": 4}, kinds
+    print(f"ranks {min(ranks)}-{max(ranks)}, ties {ties}, max error vs dense FP64 {max(final_err):.2e}")
+    assert ties == 0
+    # the rank cap nb/3 = 85 bounds the accuracy of the most coupled tiles
+    assert max(final_err) < 1e-7
+
+
+def test_tlr_cholesky_8x8_nb512_3d_matches_oracle(ctx):
+    H, O = _mods()
+    from hcorepp_amd import cholesky as HC
+    from oracle import cholesky as OC
+    from hcorepp_amd.workloads import TLRMatrix, prefetch_tiles
+    acc = 1e-8
+    M = TLRMatrix(4096, 512, 3, 0.1, acc, 20261205, nugget=1e-2)
+    prefetch_tiles(M, [(i, j) for i in range(M.T) for j in range(i + 1)], 16)
+    diag = [M.tile(k, k)[1].copy(order="F") for k in range(M.T)]
+    off = {(i, k): M.tile(i, k)[1:] for i in range(M.T) for k in range(i)}
+    print("[case] cholesky 8x8 nb=512 3-D: oracle", flush=True)
+    Fo = OC.TLRCholesky(diag, off, acc).factor()
+    Lo = Fo.dense_factor()
+    A = M.dense_block_range(0, M.n)
+    be_o = np.linalg.norm(A - Lo @ Lo.T) / np.linalg.norm(A)
+    for form in ("plain", "batched"):
+        print(f"[case] cholesky 8x8 nb=512 3-D: {form}", flush=True)
+        F = HC.TLRCholesky(ctx, M)
+        if form == "plain":
+            F.factor_plain(H.CompressionParameters(acc))
+        else:
+            F.PLAIN_STEP_MAX_UPDATES = 0
+            F.factor_batched(H.CompressionParameters(acc))
+        Ld = F.dense_factor()
+        rg, ro = F.ranks(), Fo.ranks()
+        diff = sum(abs(rg[k] - ro[k]) for k in ro)
+        be = np.linalg.norm(A - Ld @ Ld.T) / np.linalg.norm(A)
+        print(f"{form}: rank differences {diff}, backward error {be:.3e} (oracle {be_o:.3e})")
+        assert diff <= 2
+        assert rel_fro(Ld, Lo) < (1e-11 if diff == 0 else 10 * acc)
+        assert be <= 1.5 * be_o + 1e-14, (be, be_o)
+
+
+def test_learned_batch_8192_nb512_ranks_match_oracle(ctx):
+    H, O = _mods()
+    import os
+    from hcorepp_amd.learned import RankPredictor
+    from hcorepp_amd.workloads import batch_ccc
+    NB, N, MAXR, ACC = 512, 8192, 384, 1e-8
+    print("[case] learned batch 8192: generating", flush=True)
+    ups = batch_ccc(N, NB, 20261206, 8, 192, 0.85, threads=16)
+    As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
+    Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
+    arena = H.TileArena(N * H.TileArena.compressed_bytes(NB, NB, MAXR))
+    Cs = [H.CompressedTile(u["cu"], u["cv"], max_rank=MAXR, arena=arena) for u in ups]
+    b = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, H.CompressionParameters(ACC), H.PREC_FP64)
+    model = RankPredictor.load(ctx, os.path.join(os.path.dirname(H.__file__), "rank_mlp.npz"))
+    b.set_rank_model(model.bf16(), tol=0.1, margin=40)
+    print("[case] learned batch 8192: device", flush=True)
+    b.run()
+    st, _ = b.status()
+    assert st == 0
+    rk = b.ranks().copy()
+    fb, pred = b.learned_stats()
+    print("[case] learned batch 8192: oracle", flush=True)
+    oA = [O.Tile.compressed(u["au"], u["av"]) for u in ups]
+    oB = [O.Tile.compressed(u["bu"], u["bv"]) for u in ups]
+    oC = [O.Tile.compressed(u["cu"], u["cv"], max_rank=MAXR) for u in ups]
+    _, orc_ranks = O.batch_gemm(1.0, oA, oB, 1.0, oC, O.Params(ACC), 16)
+    mis = int(np.sum(rk != orc_ranks))
+    print(f"fallbacks {fb}/{N} ({100.0 * fb / N:.2f} %), rank mismatches {mis}/{N}, "
+          f"mean |predicted - rank| {np.mean(np.abs(pred - rk)):.2f}")
+    assert mis == 0
+    for i in range(0, N, N // 64):
+        Cs[i].rank = int(rk[i])
+        assert rel_fro(Cs[i].to_dense(), oC[i].to_dense()) <= 1e-8, i
+    b.close()

@@ -127,3 +127,91 @@ def test_batched_rank_rules_match_plain(ctx, kw):
             assert rk[i] == kw["fixed_rank"]
         assert rel_fro(Cb[i].to_dense(), Cp[i].to_dense()) < 1e-11
     b.close()
+
+
+def test_nan_factors_report_numeric_and_keep_c(ctx):
+    """NaN in A's factors: the plain form returns HCOREPP_ERR_NUMERIC (the device status word's
+    NaN bit), and in a batch that update reports NUMERIC through the batch status while its C
+    keeps its factors and rank; the other updates complete."""
+    H = _H()
+    rng = np.random.default_rng(41)
+    m = 96
+    au, av = _lowrank(rng, m, m, 8)
+    bu, bv = _lowrank(rng, m, m, 6)
+    cu, cv = _lowrank(rng, m, m, 10)
+    bad = au.copy()
+    bad[3, 2] = np.nan
+    A_bad, A_ok, B = H.CompressedTile(bad, av), H.CompressedTile(au, av), H.CompressedTile(bu, bv)
+    Cp = H.CompressedTile(cu, cv, max_rank=40)
+    with pytest.raises(H.HCoreError, match="NUMERIC"):
+        H.HCore.gemm(1.0, A_bad, B, 1.0, Cp, ctx, H.CompressionParameters(1e-8))
+    C0 = H.CompressedTile(cu, cv, max_rank=40)
+    C1 = H.CompressedTile(cu, cv, max_rank=40)
+    b = H.Batch(ctx, [A_bad, A_ok], [B, B], [C0, C1], 1.0, 1.0, H.CompressionParameters(1e-8))
+    b.run()
+    st, _ = b.status()
+    assert H.STATUS[st] == "NUMERIC", st
+    rk = b.sync_ranks_to_tiles()
+    assert rk[0] == 10
+    u0, v0 = C0.factors()
+    assert np.array_equal(u0, cu) and np.array_equal(v0, cv)
+    Cref = H.CompressedTile(cu, cv, max_rank=40)
+    H.HCore.gemm(1.0, A_ok, B, 1.0, Cref, ctx, H.CompressionParameters(1e-8))
+    assert rk[1] == Cref.rank
+    assert rel_fro(C1.to_dense(), Cref.to_dense()) < 1e-11
+    b.close()
+
+
+def test_batch_core_beyond_limit_keeps_c(ctx):
+    """A batched update whose stacked rank exceeds the core-SVD limit (768 rows) is refused at
+    creation; one that only exceeds it after its C rank was raised (set_ranks) reports the
+    status bit at run time and leaves C unchanged."""
+    H = _H()
+    rng = np.random.default_rng(43)
+    m = 1024
+    au, av = _lowrank(rng, m, m, 400, decay=0.999)
+    bu, bv = _lowrank(rng, m, m, 20)
+    cu, cv = _lowrank(rng, m, m, 400, decay=0.999)
+    A, B = H.CompressedTile(au, av), H.CompressedTile(bu, bv)
+    Cbig = H.CompressedTile(cu, cv, max_rank=500)
+    with pytest.raises(H.HCoreError, match="UNSUPPORTED"):
+        H.Batch(ctx, [A], [B], [Cbig], 1.0, 1.0, H.CompressionParameters(1e-8))
+    # created at rank 300 (K = 700), run at rank 400 (K = 800 > 768)
+    Cs = H.CompressedTile(cu[:, :300], cv[:300], max_rank=500)
+    b = H.Batch(ctx, [A], [B], [Cs], 1.0, 1.0, H.CompressionParameters(1e-8))
+    full = H.CompressedTile(cu, cv, max_rank=500)  # the rank-400 factors in C's buffer
+    H.lib().hcorepp_memcpy(ctx.h, Cs.buf.ptr, full.buf.ptr, full.buf.nbytes)
+    b.set_ranks(np.array([400], dtype=np.int64))
+    b.run()
+    st, _ = b.status()
+    assert H.STATUS[st] == "NUMERIC", st
+    assert b.ranks()[0] == 400
+    Cs.rank = 400
+    u, v = Cs.factors()
+    assert np.array_equal(u, cu) and np.array_equal(v, cv)
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["CCC", "DCC", "CDC", "CCD", "DDD", "DDC"])
+def test_pool_size_presizes_the_arena(kind):
+    """A context whose arena is presized with the pool-size query (HCore<T>::CalculateMemoryPoolSize,
+    reference src/api/HCore.cpp:417-480, as par_fixed_rank_streams_main.cpp:86-122 uses it) never
+    grows it for that update."""
+    H = _H()
+    rng = np.random.default_rng(51)
+    m = 256
+    au, av = _lowrank(rng, m, m, 30)
+    bu, bv = _lowrank(rng, m, m, 25)
+    cu, cv = _lowrank(rng, m, m, 40)
+    A = H.CompressedTile(au, av) if kind[0] == "C" else H.DenseTile(rng.standard_normal((m, m)))
+    B = H.CompressedTile(bu, bv) if kind[1] == "C" else H.DenseTile(rng.standard_normal((m, m)))
+    Ct = H.CompressedTile(cu, cv, max_rank=m if kind == "DDC" else 120) if kind[2] == "C" else \
+        H.DenseTile(rng.standard_normal((m, m)))
+    p = H.CompressionParameters(1e-8)
+    elems = H.lib().hcorepp_gemm_pool_size_d(C.byref(A.c()), C.byref(B.c()), C.byref(Ct.c()), C.byref(p.c()))
+    ctx = H.RunContext(0, arena_bytes=elems * 8)
+    assert ctx.arena_bytes() == elems * 8
+    H.HCore.gemm(1.0, A, B, 1.0, Ct, ctx, p)
+    ctx.sync()
+    assert ctx.arena_bytes() == elems * 8, (kind, ctx.arena_bytes(), elems * 8)
+    ctx.close()

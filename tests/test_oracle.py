@@ -368,3 +368,25 @@ def test_oracle_tlr_cholesky_planted_sign_slip_fails():
                 raise np.linalg.LinAlgError("backward error")
     finally:
         OC.syrk_compressed = orig
+
+
+def test_oracle_under_address_and_undefined_sanitizers(tmp_path):
+    """The oracle built with -fsanitize=address,undefined runs every HCore::Gemm combination, the
+    compression constructor and the tile-GEMM driver without a sanitizer report (host code only;
+    LAPACK itself is not instrumented and runs sequentially: MKL's own threads under the address
+    sanitizer returned LAPACK failures that its sequential layer does not)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "oracle_sanitize")
+    subprocess.run([gcc, "-O1", "-g", "-fopenmp", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                    "-fno-sanitize-recover=undefined", os.path.join(root, "oracle", "hcore_oracle.c"),
+                    os.path.join(root, "tests", "cpp", "oracle_sanitize.c"), "-ldl", "-lm", "-o", exe], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="halt_on_error=1",
+               OMP_NUM_THREADS="2", MKL_THREADING_LAYER="SEQUENTIAL")
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
+    assert res.returncode == 0, res.stdout + res.stderr[-3000:]
+    assert "oracle_sanitize: ok" in res.stdout
