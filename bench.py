@@ -14,7 +14,9 @@ error on a sample, ranks on all updates):
   fp64_exact    exact core (truncated only at the rounding level of its QR)
   fp64_learned  core presized by the trained BF16 rank MLP, exact acceptance test
   fp64_rule     the same tolerance applied directly to the pivoted R factor
-  mixed         factor products on FP32 matrix cores, FP32 accumulation
+  mixed_rule    factor products on FP32 matrix cores (FP32 accumulation over error-free slices
+                of the FP64 operands), core truncated by the same tolerance rule as fp64_rule
+  mixed_learned the same products with the learned core sizing of fp64_learned
 The headline `value` is the fastest form whose error stays <= 1e-8.  Also
 reported: the plain one-update-per-launch FP64 form on a sample of the same
 updates and on the nb=1024 TLR covariance GEMM, the CPU baseline (the oracle,
@@ -80,6 +82,7 @@ def alg_bytes_kind(kind, nb, ra, rb, rc, r):
 TLR_N, TLR_NB, TLR_ELL, TLR_ACC, TLR_SEED = 16384, 1024, 0.2, 1e-10, 20261203
 # C tiles of the nb=1024 TLR GEMM sample: a 4 x 4 block of off-diagonal tiles, 16 updates each
 TLR_PICKS = [(i, j) for i in range(4) for j in range(8, 12)]
+PLAIN_STREAMS = 4  # host threads / contexts of the concurrent plain leg (4 hardware queues per process)
 
 
 def host_info():
@@ -153,6 +156,32 @@ def plain_tlr_leg(H, ctx, M, picks):
             n_up += 1
     ctx.sync()
     dt = time.perf_counter() - t1
+    # the same C tiles again with PLAIN_STREAMS host threads, each on its own context (stream and
+    # arena) and its own share of the C tiles, as the reference's stream drivers run independent
+    # tiles concurrently (par_fixed_rank_streams_main.cpp:104-137)
+    import concurrent.futures as cf
+    ctxs = [H.RunContext(0) for _ in range(PLAIN_STREAMS)]
+    jobs2 = []
+    for (i, j), _, ups in jobs:
+        tc = M.tile(i, j)
+        jobs2.append((H.CompressedTile(tc[1], tc[2], max_rank=tc[3]), ups))
+
+    def run_share(t):
+        for C, ups in jobs2[t::PLAIN_STREAMS]:
+            for _, A, B in ups:
+                H.HCore.gemm(1.0, A, B, 1.0, C, ctxs[t], params)
+        ctxs[t].sync()
+
+    for c in ctxs:
+        c.sync()
+    t2 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=PLAIN_STREAMS) as ex:
+        list(ex.map(run_share, range(PLAIN_STREAMS)))
+    dt_streams = time.perf_counter() - t2
+    same = all(C2.rank == C.rank and np.allclose(C2.to_dense(), C.to_dense(), rtol=0, atol=1e-12 * max(1.0, float(
+        np.abs(C.to_dense()).max()))) for (_, C, _), (C2, _) in zip(jobs[:4], jobs2[:4]))
+    for c in ctxs:
+        c.close()
     # accuracy: each C tile against the dense FP64 result C + sum_k A(i,k) B(k,j)
     errs = []
     for (i, j), C, ups in jobs[:4]:
@@ -165,6 +194,8 @@ def plain_tlr_leg(H, ctx, M, picks):
         errs.append(float(np.linalg.norm(C.to_dense() - ref) / np.linalg.norm(ref)))
     tf = flops / dt / 1e12
     out = {"updates_per_s": round(n_up / dt, 3), "updates": n_up, "seconds": round(dt, 3),
+           f"updates_per_s_{PLAIN_STREAMS}_streams": round(n_up / dt_streams, 3),
+           "streams_results_equal_one_stream": bool(same),
            "fp64_tflops": round(tf, 4), "fp64_frac_of_peak": round(tf / PEAK_FP64_TFLOPS, 5),
            "alg_hbm_gb_per_s": round(byts / dt / 1e9, 2), "ranks_min_max": [int(min(ranks_seen)), int(max(ranks_seen))],
            "rel_err_vs_dense_fp64_max": max(errs), "rel_err_checked_c_tiles": len(errs),
@@ -175,7 +206,8 @@ def plain_tlr_leg(H, ctx, M, picks):
         b.buf.free()
     log(f"[bench] plain TLR nb={TLR_NB}: {n_up} updates in {dt:.2f}s -> {n_up / dt:.2f} updates/s, "
         f"{tf:.3f} TF/s FP64 ({100 * tf / PEAK_FP64_TFLOPS:.2f}% of peak), {byts / dt / 1e9:.1f} GB/s algorithmic; "
-        f"ranks {min(ranks_seen)}-{max(ranks_seen)}; rel err vs dense max {max(errs):.2e}")
+        f"ranks {min(ranks_seen)}-{max(ranks_seen)}; rel err vs dense max {max(errs):.2e}; with {PLAIN_STREAMS} "
+        f"streams {n_up / dt_streams:.2f} updates/s")
     return out
 
 
@@ -331,6 +363,8 @@ def cholesky_leg(H, ctx, forms):
             F.factor_batched(params, tol=LEARNED_TOL)
         elif name == "batched_mixed":
             F.factor_batched(params, precision=H.PREC_MIXED)
+        elif name == "batched_mixed_rule":
+            F.factor_batched(params, precision=H.PREC_MIXED, tol=LEARNED_TOL)
         ctx.sync()
         dt = time.perf_counter() - tt
         dL.upload(np.ascontiguousarray(F.dense_factor()).reshape(-1))
@@ -364,13 +398,13 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--forms", default="fp64_exact,fp64_learned,fp64_rule,mixed",
+    ap.add_argument("--forms", default="fp64_exact,fp64_learned,fp64_rule,mixed_rule,mixed_learned",
                     help="comma-separated forms to measure (profiling runs pick one)")
     ap.add_argument("--plain-tlr-tiles", type=int, default=len(TLR_PICKS),
                     help="C tiles of the nb=1024 TLR covariance GEMM sample timed in the plain form and on the "
                          "CPU (0 = skip)")
     ap.add_argument("--no-cpu-tlr", action="store_true", help="skip the CPU runs of the tile GEMMs")
-    ap.add_argument("--cholesky-forms", default="batched_fp64,batched_rule,batched_mixed,plain_fp64",
+    ap.add_argument("--cholesky-forms", default="batched_fp64,batched_rule,batched_mixed_rule,plain_fp64",
                     help="TLR Cholesky forms to time (empty = skip the Cholesky leg)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -457,13 +491,15 @@ def main():
     if args.no_learned:
         want -= {"fp64_learned"}
     if args.no_mixed:
-        want -= {"mixed"}
+        want -= {"mixed_rule", "mixed_learned"}
     if "fp64_exact" in want:
         measure("fp64_exact", batch)
     model_path = os.path.join(ROOT, "hcorepp_amd", "rank_mlp.npz")
-    if "fp64_learned" in want and os.path.exists(model_path):
+    model = None
+    if os.path.exists(model_path) and ({"fp64_learned", "mixed_learned"} & want):
         from hcorepp_amd.learned import RankPredictor
         model = RankPredictor.load(ctx, model_path)
+    if "fp64_learned" in want and model is not None:
         batch.set_rank_model(model.bf16(), tol=LEARNED_TOL, margin=LEARNED_MARGIN)
         f = measure("fp64_learned", batch)
         fb, pred = batch.learned_stats()
@@ -474,11 +510,19 @@ def main():
         batch.set_rank_model(None, tol=LEARNED_TOL, margin=0)
         measure("fp64_rule", batch)
     batch.set_rank_model(None, tol=0.0, margin=0)
-    if "mixed" in want:
+    if {"mixed_rule", "mixed_learned"} & want:
         bm = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, params, H.PREC_MIXED)
         if use_graph:
             bm.capture()
-        measure("mixed", bm)
+        if "mixed_rule" in want:
+            bm.set_rank_model(None, tol=LEARNED_TOL, margin=0)
+            measure("mixed_rule", bm)
+        if "mixed_learned" in want and model is not None:
+            bm.set_rank_model(model.bf16(), tol=LEARNED_TOL, margin=LEARNED_MARGIN)
+            f = measure("mixed_learned", bm)
+            fb, pred = bm.learned_stats()
+            f["fallbacks"] = fb
+            f["mae"] = float(np.mean(np.abs(pred - f["ranks"])))
         bm.close()
 
     # ---- plain one-update-per-launch FP64 form on a sample of the same workload
@@ -549,8 +593,8 @@ def main():
         dense = ups[i]["cu"] @ ups[i]["cv"] + ups[i]["au"] @ (ups[i]["av"] @ ups[i]["bu"]) @ ups[i]["bv"]
         errs_dense.append(np.linalg.norm(u_ @ v_ - dense) / np.linalg.norm(dense))
 
-    # headline: the fastest form within the metric's error bound
-    valid = {k: f for k, f in forms.items() if f["err"] <= 1e-8}
+    # headline: the fastest form within the metric's error bound whose ranks all equal the oracle's
+    valid = {k: f for k, f in forms.items() if f["err"] <= 1e-8 and f["rank_mis"] == 0}
     best = max(valid, key=lambda k: valid[k]["ups"])
     hb = forms[best]
     ranks = hb["ranks"]
@@ -584,7 +628,11 @@ def main():
                         f"the dropped part <= {LEARNED_TOL} * accuracy, else exact core), persistent kernel",
         "fp64_rule": f"batched FP64, core truncated where the pivoted R factor's dropped part <= {LEARNED_TOL} * "
                      f"accuracy, persistent kernel",
-        "mixed": "batched, factor products on FP32 matrix cores with FP32 accumulation, QR/SVD/truncation FP64",
+        "mixed_rule": "batched, factor products on FP32 matrix cores with FP32 accumulation over error-free slices "
+                      f"of the FP64 operands, QR/SVD/truncation FP64, core truncated as fp64_rule (tol {LEARNED_TOL}), "
+                      "persistent kernel",
+        "mixed_learned": "batched, mixed-precision factor products as mixed_rule, core presized by the BF16 rank MLP "
+                         f"(margin {LEARNED_MARGIN}, tol {LEARNED_TOL}, else exact core), persistent kernel",
     }
 
     def form_json(k, f):

@@ -153,28 +153,80 @@ __device__ void wg_gemm(int M, int N, int Kd, T alpha, const T *A, int64_t lda, 
 
 // ---------------------------------------------------------------------------
 // Mixed-precision tile GEMM: C = op(A) op(B) (alpha = 1, beta = 0) for FP64 operands with the
-// products formed on the FP32 matrix cores (v_mfma_f32_16x16x4f32) and accumulated in FP32.
+// products formed on the FP32 matrix cores (v_mfma_f32_16x16x4f32) with FP32 accumulation, made
+// exact by an error-free split of the operands (the Ozaki scheme).  Each row i of op(A) is scaled
+// by a power of two 2^ea_i above its largest magnitude and cut into MS_SLICES slices: integers
+// x_s of at most MS_BITS bits with op(A)[i, k] = 2^ea_i sum_s x_s 2^(-MS_BITS (s+1)) + (a rest
+// below 2^(ea_i - MS_BITS MS_SLICES - 1)); each column of op(B) likewise.  A slice product summed
+// over one 16-deep K chunk is an integer of magnitude at most 16 * 2^(2 MS_BITS) = 2^24, exact
+// in FP32 whatever order the matrix core sums in; every slice pair (s, t) with s + t < MS_SLICES
+// is summed per chunk in FP32 and added to the FP64 result with its power-of-two weight.  The
+// error is the dropped slices: about 2^-40 of 2^(ea_i + eb_j) per term (FP64 rounding for the
+// rest), against 2^-24 for plain FP32 operands.
 // A 64 x 64 output tile per workgroup: wave w computes the 32 x 32 quarter at rows 32 (w / 2),
-// columns 32 (w % 2) as 2 x 2 blocks of 16 x 16.  K is staged through LDS as FP32 in chunks of 16.
+// columns 32 (w % 2) as 2 x 2 blocks of 16 x 16.
 // MFMA lane layout (16x16x4): A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k,
 // D[4 (l / 16) + v][l % 16] in element v of lane l.
 // ---------------------------------------------------------------------------
 typedef float hcamd_floatx4 __attribute__((ext_vector_type(4)));
-constexpr int GTM_LDS_FLOATS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
+constexpr int MS_BITS = 10, MS_SLICES = 4;
+constexpr int GTM_SLICE_A = GT_K * (GT_M + 1), GTM_SLICE_B = GT_K * (GT_N + 1);
+constexpr int GTM_LDS_FLOATS = MS_SLICES * (GTM_SLICE_A + GTM_SLICE_B) + GT_M + GT_N;
+static_assert(2 * MS_BITS + 4 <= 24, "slice products over a 16-deep chunk must be exact in FP32");
+
+// Slices of v in [-1, 1] (a value divided by its row's or column's power-of-two scale).
+__device__ __forceinline__ void ms_split(double v, float (&x)[MS_SLICES]) {
+    double r = v;
+#pragma unroll
+    for (int s = 0; s < MS_SLICES; ++s) {
+        const double q = rint(ldexp(r, MS_BITS * (s + 1)));
+        x[s] = (float) q;
+        r -= ldexp(q, -MS_BITS * (s + 1));
+    }
+}
+
+// Exponent e with |v| < 2^e for the largest magnitude v of a row or column (0 for a zero one).
+__device__ __forceinline__ int ms_exponent(double vmax) {
+    return vmax > 0.0 ? ilogb(vmax) + 1 : 0;
+}
 
 __device__ inline void wg_gemm_tile_mixed(int M, int N, int Kd, const double *A, int64_t lda, bool ta,
                                           const double *B, int64_t ldb, bool tb, double *C, int64_t ldc, int ti,
                                           int tj, float *lds) {
-    float *As = lds;                      // As[k][i]
-    float *Bs = lds + GT_K * (GT_M + 1);  // Bs[k][j]
+    float *As = lds;                                   // [slice][k][i], GTM_SLICE_A per slice
+    float *Bs = lds + MS_SLICES * GTM_SLICE_A;         // [slice][k][j]
+    int *ea = reinterpret_cast<int *>(Bs + MS_SLICES * GTM_SLICE_B);  // row exponents of op(A)
+    int *eb = ea + GT_M;                                               // column exponents of op(B)
     const int i0 = ti * GT_M, j0 = tj * GT_N;
     const int l = lane_id(), w = wave_id();
     const int wr = 32 * (w / 2), wc = 32 * (w % 2);
-    hcamd_floatx4 acc[2][2];
+    auto opA = [&](int i, int k) -> double { return ta ? A[k + (int64_t) i * lda] : A[i + (int64_t) k * lda]; };
+    auto opB = [&](int k, int j) -> double { return tb ? B[j + (int64_t) k * ldb] : B[k + (int64_t) j * ldb]; };
+    // power-of-two scales: four threads per row of op(A) (then per column of op(B)) scan K
+    {
+        const int idx = threadIdx.x / 4, part = threadIdx.x % 4;
+        double amax = 0.0, bmax = 0.0;
+        if (i0 + idx < M)
+            for (int k = part; k < Kd; k += 4) amax = fmax(amax, fabs(opA(i0 + idx, k)));
+        if (j0 + idx < N)
+            for (int k = part; k < Kd; k += 4) bmax = fmax(bmax, fabs(opB(k, j0 + idx)));
+        amax = fmax(amax, __shfl_xor(amax, 1, WAVE));
+        amax = fmax(amax, __shfl_xor(amax, 2, WAVE));
+        bmax = fmax(bmax, __shfl_xor(bmax, 1, WAVE));
+        bmax = fmax(bmax, __shfl_xor(bmax, 2, WAVE));
+        if (part == 0) {
+            ea[idx] = ms_exponent(amax);
+            eb[idx] = ms_exponent(bmax);
+        }
+    }
+    double acc[2][2][4];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) acc[p][q] = hcamd_floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[p][q][v] = 0.0;
+    __syncthreads();
     for (int k0 = 0; k0 < Kd; k0 += GT_K) {
         for (int e = threadIdx.x; e < GT_M * GT_K; e += NT) {
             int i, k;
@@ -186,9 +238,11 @@ __device__ inline void wg_gemm_tile_mixed(int M, int N, int Kd, const double *A,
                 k = e / GT_M;
             }
             const int gi = i0 + i, gk = k0 + k;
-            double v = 0.0;
-            if (gi < M && gk < Kd) v = ta ? A[gk + (int64_t) gi * lda] : A[gi + (int64_t) gk * lda];
-            As[k * (GT_M + 1) + i] = (float) v;
+            const double v = (gi < M && gk < Kd) ? ldexp(opA(gi, gk), -ea[i]) : 0.0;
+            float x[MS_SLICES];
+            ms_split(v, x);
+#pragma unroll
+            for (int sl = 0; sl < MS_SLICES; ++sl) As[sl * GTM_SLICE_A + k * (GT_M + 1) + i] = x[sl];
         }
         for (int e = threadIdx.x; e < GT_N * GT_K; e += NT) {
             int j, k;
@@ -200,23 +254,44 @@ __device__ inline void wg_gemm_tile_mixed(int M, int N, int Kd, const double *A,
                 j = e / GT_K;
             }
             const int gj = j0 + j, gk = k0 + k;
-            double v = 0.0;
-            if (gj < N && gk < Kd) v = tb ? B[gj + (int64_t) gk * ldb] : B[gk + (int64_t) gj * ldb];
-            Bs[k * (GT_N + 1) + j] = (float) v;
+            const double v = (gj < N && gk < Kd) ? ldexp(opB(gk, gj), -eb[j]) : 0.0;
+            float x[MS_SLICES];
+            ms_split(v, x);
+#pragma unroll
+            for (int sl = 0; sl < MS_SLICES; ++sl) Bs[sl * GTM_SLICE_B + k * (GT_N + 1) + j] = x[sl];
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GT_K; kk += 4) {
-            const int k = kk + l / 16;
-            float a[2], b[2];
+        for (int sa = 0; sa < MS_SLICES; ++sa) {
 #pragma unroll
-            for (int p = 0; p < 2; ++p) a[p] = As[k * (GT_M + 1) + wr + 16 * p + (l % 16)];
+            for (int sb = 0; sb + sa < MS_SLICES; ++sb) {
+                hcamd_floatx4 part[2][2];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) b[q] = Bs[k * (GT_N + 1) + wc + 16 * q + (l % 16)];
+                for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
+                    for (int q = 0; q < 2; ++q) part[p][q] = hcamd_floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int q = 0; q < 2; ++q) acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p], b[q], acc[p][q], 0, 0, 0);
+                for (int kk = 0; kk < GT_K; kk += 4) {
+                    const int k = kk + l / 16;
+                    float a[2], b[2];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) a[p] = As[sa * GTM_SLICE_A + k * (GT_M + 1) + wr + 16 * p + (l % 16)];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) b[q] = Bs[sb * GTM_SLICE_B + k * (GT_N + 1) + wc + 16 * q + (l % 16)];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            part[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p], b[q], part[p][q], 0, 0, 0);
+                }
+                const double wgt = ldexp(1.0, -MS_BITS * (sa + sb + 2));
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[p][q][v] += (double) part[p][q][v] * wgt;
+            }
         }
         __syncthreads();
     }
@@ -226,10 +301,11 @@ __device__ inline void wg_gemm_tile_mixed(int M, int N, int Kd, const double *A,
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int gi = i0 + wr + 16 * p + 4 * (l / 16) + v;
-                const int gj = j0 + wc + 16 * q + (l % 16);
-                if (gi < M && gj < N) C[gi + (int64_t) gj * ldc] = (double) acc[p][q][v];
+                const int ri = wr + 16 * p + 4 * (l / 16) + v, cj = wc + 16 * q + (l % 16);
+                const int gi = i0 + ri, gj = j0 + cj;
+                if (gi < M && gj < N) C[gi + (int64_t) gj * ldc] = ldexp(acc[p][q][v], ea[ri] + eb[cj]);
             }
+    __syncthreads();
 }
 
 __device__ inline void wg_gemm_mixed(int M, int N, int Kd, const double *A, int64_t lda, bool ta, const double *B,

@@ -365,8 +365,11 @@ hipError_t launch_fill_identity(hipStream_t s, T *I, int64_t n) {
 // ---------------------------------------------------------------------------
 // Batched persistent kernel.
 // ---------------------------------------------------------------------------
-// Factor products of one update (C.Gemm's temporaries).  mixed: FP32 matrix-core products
-// with FP32 accumulation (FP64 operands and results), otherwise FP64.
+static_assert(GTM_LDS_FLOATS * sizeof(float) <= (size_t) BATCH_LDS_BYTES, "mixed tile GEMM LDS");
+
+// Factor products of one update (C.Gemm's temporaries).  mixed: FP32 matrix-core products of
+// the operands' error-free slices with FP32 accumulation (FP64 operands and results,
+// wg_gemm_mixed), otherwise FP64.
 template <typename T>
 __device__ void run_temp_gemms(const BatchEntry<T> &e, T *lds, int mixed) {
     for (int g = 0; g < e.ngemm; ++g) {

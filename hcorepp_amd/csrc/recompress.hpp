@@ -313,9 +313,82 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, i
     apply_wy<T, PB, RPT>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
 }
 
+// ---------------------------------------------------------------------------
+// The same block-reflector application B[:, c0:c1] <- (I - P op(T) P^T) B for 16-column FP64
+// panels, with both products on the matrix cores (v_mfma_f64_16x16x4f64).  Each wave owns whole
+// 16-column blocks of B, so the reduction over the rows happens inside the matrix core and no
+// barrier or cross-wave exchange is needed:
+//   W  = P^T B_blk    (16 x 16, K = rows: rows/4 MFMA steps; A = P^T from LDS, B from memory)
+//   W' = op(T) W      (VALU: W's columns gathered with shuffles, T from LDS)
+//   B_blk -= P W'     (per 16 rows: 4 MFMA steps of K = 4; A = P from LDS, B = W')
+// MFMA lane layout (f64 16x16x4): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
+// D[4 v + l / 16][l % 16] in element v of lane l (row groups of one, unlike the FP32 16x16x4
+// instruction's groups of four).  P: panel in LDS (rows x 16, ld pld, unit lower); Tm: 16 x 16
+// upper (LDS).
+// ---------------------------------------------------------------------------
+typedef double hcamd_doublex4 __attribute__((ext_vector_type(4)));
+
+__device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const double *Tm, int rows, double *B,
+                                               int64_t ldb, int c0, int c1, bool trans) {
+    const int l = lane_id(), i16 = l & 15, k4 = l >> 4;
+    const int nblk = (c1 - c0 + 15) / 16;
+    for (int blk = wave_id(); blk < nblk; blk += NWAVES) {
+        const int col = c0 + 16 * blk + i16;
+        const bool cok = col < c1;
+        double *bcol = B + (int64_t) (cok ? col : c0) * ldb;
+        hcamd_doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < rows; k0 += 4) {
+            const int r = k0 + k4;
+            const double a = r < rows ? P[r + i16 * pld] : 0.0;
+            const double b = (r < rows && cok) ? bcol[r] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        // acc[v] = W[4 v + k4][i16].  Gather column i16 of W: W[p][i16] is element p / 4 of lane
+        // i16 + 16 (p % 4).
+        double wc[16];
+#pragma unroll
+        for (int p = 0; p < 16; ++p) wc[p] = __shfl(acc[p >> 2], i16 + 16 * (p & 3), WAVE);
+        // bop[s] = W'[4 s + k4][i16], the B operand of step s of the update
+        double bop[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int i = 4 * s + k4;
+            double t = 0.0;
+            if (trans) {
+#pragma unroll
+                for (int p = 0; p < 16; ++p) t += (p <= i) ? Tm[p + 16 * i] * wc[p] : 0.0;
+            } else {
+#pragma unroll
+                for (int p = 0; p < 16; ++p) t += (p >= i) ? Tm[i + 16 * p] * wc[p] : 0.0;
+            }
+            bop[s] = t;
+        }
+        for (int r0 = 0; r0 < rows; r0 += 16) {
+            hcamd_doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int r = r0 + i16;
+                const double a = r < rows ? P[r + (4 * s + k4) * pld] : 0.0;
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bop[s], d, 0, 0, 0);
+            }
+            if (cok) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = r0 + 4 * v + k4;
+                    if (r < rows) bcol[r] -= d[v];
+                }
+            }
+        }
+    }
+}
+
 template <typename T, int PB>
 __device__ void apply_wy_dispatch(const T *P, int pld, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
                                   bool trans, T *red) {
+    if constexpr (PB == 16 && sizeof(T) == 8) {
+        apply_wy_mfma(P, pld, Tm, rows, B, ldb, c0, c1, trans);
+        return;
+    }
     // rows <= 512 for PB 16, <= 1024 for PB 8, <= 2048 for PB 4 (pick_pb): at most 32 panel values per lane
     const int chunks = (rows + WAVE * NWAVES - 1) / (WAVE * NWAVES);
     if (chunks <= 1) apply_wy_nl<T, PB, 1>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);

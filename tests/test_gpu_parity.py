@@ -246,10 +246,12 @@ def test_batched_matches_plain(ctx):
         assert rel_fro(Cs_b[i].to_dense(), Cs_p[i].to_dense()) < 1e-12
 
 
-def test_batched_mixed_precision_error_is_fp32_level(ctx):
-    """Mixed form: factor products on FP32 matrix cores with FP32 accumulation.  Its result differs
-    from the FP64 oracle by FP32 rounding of the A*B term (rel err well below 1e-5 and above FP64
-    rounding), while the FP64 batched form of the same inputs stays at FP64 rounding."""
+def test_batched_mixed_precision_meets_1e8(ctx):
+    """Mixed form: the factor products on FP32 matrix cores with FP32 accumulation, over the
+    operands' error-free slices (10-bit integer slices per power-of-two row/column scale, exact
+    FP32 partial sums, FP64 sums of the slice pairs).  Its result stays within the form's 1e-8
+    bar of the FP64 oracle (the dropped slices are ~2^-40 of each term), and the FP64 batched form
+    of the same inputs at FP64 rounding."""
     H, O = _H(), _orc()
     from hcorepp_amd.workloads import batch_ccc
     ups = batch_ccc(16, 256, seed=7, lo=8, hi=96)
@@ -272,7 +274,8 @@ def test_batched_mixed_precision_error_is_fp32_level(ctx):
         res[prec] = max(errs)
         b.close()
     assert res[H.PREC_FP64] < 1e-12
-    assert 1e-12 < res[H.PREC_MIXED] < 1e-5
+    # FP32 matrix-core products of the error-free operand slices: within the form's 1e-8 bar
+    assert res[H.PREC_MIXED] < 1e-8, res[H.PREC_MIXED]
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
