@@ -56,7 +56,8 @@ EXPORTED = [
     "hcorepp_ctx_arena_bytes", "hcorepp_ctx_compute_units", "hcorepp_ctx_event_record", "hcorepp_ctx_event_elapsed", "hcorepp_malloc", "hcorepp_free", "hcorepp_memcpy",
     "hcorepp_memcpy_async", "hcorepp_memset", "hcorepp_kernel_gemm_d", "hcorepp_compress_d", "hcorepp_compress_s",
     "hcorepp_gemm_d", "hcorepp_gemm_s", "hcorepp_gemm_pool_size_d", "hcorepp_gemm_pool_size_s",
-    "hcorepp_tile_potrf_d", "hcorepp_tile_potrf_s", "hcorepp_tile_trsm_d",
+    "hcorepp_gemm_noalloc_d",
+    "hcorepp_gemm_noalloc_scratch_bytes", "hcorepp_tile_potrf_d", "hcorepp_tile_potrf_s", "hcorepp_tile_trsm_d",
     "hcorepp_tile_trsm_s", "hcorepp_tile_syrk_d", "hcorepp_tile_syrk_s",
     "hcorepp_batch_create", "hcorepp_batch_run", "hcorepp_batch_capture_graph", "hcorepp_batch_launch_graph",
     "hcorepp_batch_rank_dev", "hcorepp_batch_ranks", "hcorepp_batch_set_ranks", "hcorepp_batch_status",
@@ -122,6 +123,9 @@ def lib():
     L.hcorepp_gemm_s.argtypes = [P, C.c_float, C.POINTER(_Tile), C.c_char, C.POINTER(_Tile), C.c_char,
                                  C.c_float, C.POINTER(_Tile), C.POINTER(_CParams), C.c_int,
                                  C.POINTER(C.c_uint64), P, C.POINTER(I64)]
+    L.hcorepp_gemm_noalloc_d.argtypes = [P, D, C.POINTER(_Tile), C.c_char, C.POINTER(_Tile), C.c_char, D,
+                                         C.POINTER(_Tile), C.POINTER(_CParams), C.POINTER(C.c_uint64)]
+    L.hcorepp_gemm_noalloc_scratch_bytes.restype = C.c_size_t
     for t, F in (("d", D), ("s", C.c_float)):
         getattr(L, f"hcorepp_tile_potrf_{t}").argtypes = [P, C.c_char, C.POINTER(_Tile)]
         getattr(L, f"hcorepp_tile_trsm_{t}").argtypes = [P, C.c_char, C.c_char, C.c_char, C.c_char, F,

@@ -62,6 +62,10 @@ struct hcorepp_ctx {
     // last plain recompression (diagnostics: hcorepp_debug_workspace)
     void *last_ws = nullptr;
     int64_t last_info[16] = {};
+    // while set (hcorepp_gemm_noalloc_d), the arena functions carve this fixed device region
+    // instead of the context's own arena, and never allocate
+    char *fixed = nullptr;
+    size_t fixed_cap = 0;
 };
 
 namespace {
@@ -86,6 +90,12 @@ hcorepp_status ensure_lds_limits() {
 // stream) when the request exceeds the current capacity, then resets the bump pointer.
 hcorepp_status arena_begin(hcorepp_ctx *ctx, size_t bytes) {
     ctx->arena_off = 0;
+    if (ctx->fixed) {
+        if (bytes > ctx->fixed_cap)
+            return fail(HCOREPP_ERR_CAPACITY, "the update needs %zu bytes of device scratch; the allocation-free "
+                        "entry point has %zu", bytes, ctx->fixed_cap);
+        return HCOREPP_OK;
+    }
     if (bytes <= ctx->arena_cap) return HCOREPP_OK;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->arena) HIP_TRY(hipFree(ctx->arena));
@@ -103,8 +113,53 @@ hcorepp_status arena_begin(hcorepp_ctx *ctx, size_t bytes) {
 void *arena_take(hcorepp_ctx *ctx, size_t bytes) {
     size_t off = (ctx->arena_off + 255) & ~size_t(255);
     ctx->arena_off = off + bytes;
-    return ctx->arena + off;
+    return (ctx->fixed ? ctx->fixed : ctx->arena) + off;
 }
+
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry point (hcorepp_gemm_noalloc_d).  Every GPU the
+// library runs on holds its own copy of this array until the process ends.  In host code the
+// array's name is only a stand-in: its device address is looked up once per device
+// (hipGetSymbolAddress, with that device current, when the first context on it is created) and
+// only that address is handed to kernels.  All calls on one GPU share its array; calls that
+// overlap on one GPU race.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) static unsigned char scratch[kScratchBytes];
+char *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// Look up (once per device, on the first allocation-free call there) the address of `scratch` on
+// `device`, with that device current.
+hcorepp_status scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(HCOREPP_ERR_INVALID, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return HCOREPP_OK;
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) HIP_TRY(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess) return fail(HCOREPP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    if (!p) return fail(HCOREPP_ERR_HIP, "no device address for the scratch array on device %d", device);
+    g_scratch_ptr[device] = static_cast<char *>(p);
+    return HCOREPP_OK;
+}
+
+// Points a context's arena functions at its device's scratch array for one call.
+struct FixedArena {
+    hcorepp_ctx *ctx;
+    FixedArena(hcorepp_ctx *c, char *base, size_t cap) : ctx(c) {
+        ctx->fixed = base;
+        ctx->fixed_cap = cap;
+    }
+    ~FixedArena() {
+        ctx->fixed = nullptr;
+        ctx->fixed_cap = 0;
+        ctx->arena_off = 0;
+    }
+};
 
 size_t aligned(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -806,6 +861,18 @@ hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile 
                               int64_t *n_sigma) {
     return gemm_plain<float>(ctx, alpha, A, opA, B, opB, beta, C, params, cholesky, flops, sigma_out, n_sigma);
 }
+
+hcorepp_status hcorepp_gemm_noalloc_d(hcorepp_ctx *ctx, double alpha, const hcorepp_tile *A, char opA,
+                                      const hcorepp_tile *B, char opB, double beta, hcorepp_tile *C,
+                                      const hcorepp_cparams *params, uint64_t *flops) {
+    if (!ctx) return fail(HCOREPP_ERR_INVALID, "null context");
+    hcorepp_status st = scratch_init(ctx->device);
+    if (st) return st;
+    FixedArena guard(ctx, g_scratch_ptr[ctx->device], kScratchBytes);
+    return gemm_plain<double>(ctx, alpha, A, opA, B, opB, beta, C, params, 0, flops, nullptr, nullptr);
+}
+
+size_t hcorepp_gemm_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
 hcorepp_status hcorepp_tile_potrf_d(hcorepp_ctx *ctx, char uplo, hcorepp_tile *A) {
     return tile_potrf<double>(ctx, uplo, A);

@@ -146,6 +146,23 @@ hcorepp_status hcorepp_gemm_s(hcorepp_ctx *ctx, float alpha, const hcorepp_tile 
                               const hcorepp_cparams *params, int cholesky, uint64_t *flops,
                               double *sigma_out, int64_t *n_sigma);
 
+/* Allocation-free variant of hcorepp_gemm_d (FP64, no cholesky, no sigma output).  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Instead of
+ * the context arena, every kernel's scratch (temporary factor products, the recompression
+ * workspace, panel and flag buffers) lives in a fixed device array of
+ * hcorepp_gemm_noalloc_scratch_bytes() bytes that each GPU holds for the life of the process;
+ * the call never allocates or frees device memory.  An update that needs more scratch returns
+ * HCOREPP_ERR_CAPACITY (16 MiB covers nb = 512 tiles with stacked ranks up to about 400).
+ * All calls on one GPU share that GPU's array, which keeps the last call's contents: calls that
+ * overlap on one GPU (several contexts or threads) race and must be serialised by the caller.
+ * Same results as hcorepp_gemm_d; it synchronises the context stream once, as that does. */
+hcorepp_status hcorepp_gemm_noalloc_d(hcorepp_ctx *ctx, double alpha, const hcorepp_tile *A, char opA,
+                                      const hcorepp_tile *B, char opB, double beta, hcorepp_tile *C,
+                                      const hcorepp_cparams *params, uint64_t *flops);
+/* Size in bytes of the per-GPU scratch array of hcorepp_gemm_noalloc_d. */
+size_t hcorepp_gemm_noalloc_scratch_bytes(void);
+
 /* Compression constructor on the device: replaces CompressedTile<T>(rows, cols, apData, ld,
  * CompressionParameters, layout, context) (reference src/operators/concrete/Compressed.cpp:75-146).
  * A: device pointer, rows x cols column-major (ld lda).  out: compressed tile descriptor with
