@@ -404,6 +404,8 @@ def main():
                     help="C tiles of the nb=1024 TLR covariance GEMM sample timed in the plain form and on the "
                          "CPU (0 = skip)")
     ap.add_argument("--no-cpu-tlr", action="store_true", help="skip the CPU runs of the tile GEMMs")
+    ap.add_argument("--learned-margin", type=int, default=LEARNED_MARGIN,
+                    help="columns added to the rank model's prediction when presizing the core")
     ap.add_argument("--cholesky-forms", default="batched_fp64,batched_rule,batched_mixed_rule,plain_fp64",
                     help="TLR Cholesky forms to time (empty = skip the Cholesky leg)")
     args = ap.parse_args()
@@ -500,7 +502,7 @@ def main():
         from hcorepp_amd.learned import RankPredictor
         model = RankPredictor.load(ctx, model_path)
     if "fp64_learned" in want and model is not None:
-        batch.set_rank_model(model.bf16(), tol=LEARNED_TOL, margin=LEARNED_MARGIN)
+        batch.set_rank_model(model.bf16(), tol=LEARNED_TOL, margin=args.learned_margin)
         f = measure("fp64_learned", batch)
         fb, pred = batch.learned_stats()
         f["fallbacks"] = fb
@@ -518,7 +520,7 @@ def main():
             bm.set_rank_model(None, tol=LEARNED_TOL, margin=0)
             measure("mixed_rule", bm)
         if "mixed_learned" in want and model is not None:
-            bm.set_rank_model(model.bf16(), tol=LEARNED_TOL, margin=LEARNED_MARGIN)
+            bm.set_rank_model(model.bf16(), tol=LEARNED_TOL, margin=args.learned_margin)
             f = measure("mixed_learned", bm)
             fb, pred = bm.learned_stats()
             f["fallbacks"] = fb
@@ -614,17 +616,23 @@ def main():
     # executed FP64 VALU flops and matrix-core busy fraction of this form (rocprofv3 SQ counters;
     # quoted from the committed summary, which gives the command)
     executed, mfma_busy, counters_src = None, None, None
+    exec_valu, exec_mfma = None, None
     cprof = os.path.join(ROOT, "profiles", f"k_batched_ccc{args.updates}_nb{NB}_valu_mfma_counters.md")
     if os.path.exists(cprof):
         for line in open(cprof):
             if line.startswith(f"{best}.valu_fp64_flop_per_launch:"):
-                executed = float(line.split(":")[1])
+                exec_valu = float(line.split(":")[1])
                 counters_src = f"rocprofv3 PMC, {os.path.relpath(cprof, ROOT)}"
+            elif line.startswith(f"{best}.mfma_f64_flop_per_launch:"):
+                exec_mfma = float(line.split(":")[1])
             elif line.startswith(f"{best}.mfma_busy_frac:"):
                 mfma_busy = float(line.split(":")[1])
+    if exec_valu is not None:
+        # FP64 work the kernel executes: VALU FP64 instructions plus FP64 matrix-core operations
+        executed = exec_valu + (exec_mfma or 0.0)
     describe = {
         "fp64_exact": "batched FP64, exact core, persistent kernel",
-        "fp64_learned": f"batched FP64, core presized by the BF16 rank MLP (margin {LEARNED_MARGIN}, accepted when "
+        "fp64_learned": f"batched FP64, core presized by the BF16 rank MLP (margin {args.learned_margin}, accepted when "
                         f"the dropped part <= {LEARNED_TOL} * accuracy, else exact core), persistent kernel",
         "fp64_rule": f"batched FP64, core truncated where the pivoted R factor's dropped part <= {LEARNED_TOL} * "
                      f"accuracy, persistent kernel",
@@ -632,7 +640,7 @@ def main():
                       f"of the FP64 operands, QR/SVD/truncation FP64, core truncated as fp64_rule (tol {LEARNED_TOL}), "
                       "persistent kernel",
         "mixed_learned": "batched, mixed-precision factor products as mixed_rule, core presized by the BF16 rank MLP "
-                         f"(margin {LEARNED_MARGIN}, tol {LEARNED_TOL}, else exact core), persistent kernel",
+                         f"(margin {args.learned_margin}, tol {LEARNED_TOL}, else exact core), persistent kernel",
     }
 
     def form_json(k, f):
@@ -679,6 +687,9 @@ def main():
         "tlr_cholesky_nb512": chol,
         "roofline": {
             "bound": "mfma",
+            "bound_note": "FP64 compute roof (dense FP64 matrix-core peak); the kernel's HBM traffic is "
+                          "traffic_gb_per_s against 8000 GB/s, so neither roof is reached: the limit is the "
+                          "latency of the dependent Householder/Jacobi steps per update",
             "achieved": round(achieved_tflops, 4),
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -687,6 +698,8 @@ def main():
             "traffic_source": traffic_src,
             "traffic_gb_per_s": round(traffic / (hb["kernel_ms"] * 1e-3) / 1e9, 1) if traffic else None,
             "executed_fp64_flop_per_launch": executed,
+            "executed_fp64_valu_flop_per_launch": exec_valu,
+            "executed_fp64_mfma_flop_per_launch": exec_mfma,
             "executed_fp64_tflops": round(executed / (hb["kernel_ms"] * 1e-3) / 1e12, 4) if executed else None,
             "mfma_busy_frac": mfma_busy,
             "counters_source": counters_src,

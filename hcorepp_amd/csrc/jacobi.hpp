@@ -18,8 +18,11 @@
 // singular values).  A sweep ends the iteration when every pair it rotated had
 // a cosine of at most JacobiStopCos before its rotation: Jacobi converges
 // quadratically, so the cosines it leaves are of the order of that bound's
-// square (U orthonormal to within 1e-10, singular values exact to rounding), and
-// the sweeps that would only shrink them further are skipped.
+// square, and the sweeps that would only shrink them further are skipped.  The
+// forms that accumulate V (the plain, one-update-at-a-time form) stop at
+// 0.1 sqrt(eps), which leaves U orthonormal to rounding as LAPACK's SVD does; the
+// batched form without V stops at 1e-5 in FP64 (cosines left of order 1e-10, U
+// orthonormal to about 1e-10, singular values exact to rounding), one sweep fewer.
 #pragma once
 
 #include "device_common.hpp"
@@ -40,11 +43,13 @@ template <typename T>
 struct JacobiStopCos;
 template <>
 struct JacobiStopCos<double> {
-    static constexpr double value = 1e-5;
+    static constexpr double tight = 1.4901161193847656e-09;  // 0.1 * sqrt(2^-52)
+    static constexpr double loose = 1e-5;
 };
 template <>
 struct JacobiStopCos<float> {
-    static constexpr float value = 3.4526698e-05f;  // 0.1 * sqrt(2^-23)
+    static constexpr float tight = 3.4526698e-05f;  // 0.1 * sqrt(2^-23)
+    static constexpr float loose = 3.4526698e-05f;
 };
 
 // Lane that holds value index j after wave_multi_sum<NV>: index bits map to lane bits 5..(6-log2 NV).
@@ -80,13 +85,14 @@ __host__ __device__ __forceinline__ constexpr int jpair_q(int rd, int k) {
 // column.  t = tan(theta) is the smaller root of t^2 + 2 zeta t - 1 = 0 with
 // zeta = (aqq - app) / (2 apq), written as t = sign(d) e / (|d| + hypot(d, e)) for d = aqq - app,
 // e = 2 apq (one division and one square root); c = 1/sqrt(1 + t^2), s = c t.
-template <typename T>
+// TIGHT selects the stop cosine (JacobiStopCos: tight for the forms that accumulate V).
+template <typename T, bool TIGHT>
 __device__ __forceinline__ void jacobi_rotation(T app, T aqq, T apq, T tol, T floor2, T &c, T &s, int &large) {
     c = T(1);
     s = T(0);
     large = 0;
     if (app > floor2 && aqq > floor2 && fabs(apq) > tol * sqrt(app * aqq)) {
-        large = fabs(apq) > JacobiStopCos<T>::value * sqrt(app * aqq);
+        large = fabs(apq) > (TIGHT ? JacobiStopCos<T>::tight : JacobiStopCos<T>::loose) * sqrt(app * aqq);
         const T d = aqq - app, e = T(2) * apq;
         // scale by a power of two so that d*d + e*e neither overflows nor underflows
         const int ex = ilogb(fmax(fabs(d), fabs(e)));
@@ -135,7 +141,7 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>
     const T apq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 2), WAVE);
     T c, s;
     int lg;
-    jacobi_rotation(app, aqq, apq, tol, floor2, c, s, lg);
+    jacobi_rotation<T, (RPLV > 0)>(app, aqq, apq, tol, floor2, c, s, lg);
     const unsigned long long lgm = __ballot(l < NP && lg);
     int any = 0;
 #pragma unroll
@@ -321,7 +327,7 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
             aqq += buf[w2 * NV + 3 * l + 1];
             apq += buf[w2 * NV + 3 * l + 2];
         }
-        jacobi_rotation(app, aqq, apq, tol, floor2, c, s, lg);
+        jacobi_rotation<T, (RPLV > 0)>(app, aqq, apq, tol, floor2, c, s, lg);
     }
     const unsigned long long lgm = __ballot(lg);
     int any = 0;
@@ -455,7 +461,7 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
             aqq += buf[w2 * NV + 3 * l + 1];
             apq += buf[w2 * NV + 3 * l + 2];
         }
-        jacobi_rotation(app, aqq, apq, tol, floor2, c, s, lg);
+        jacobi_rotation<T, true>(app, aqq, apq, tol, floor2, c, s, lg);
         if (w == 0) {
             cs[2 * (rd * NP + l)] = c;
             cs[2 * (rd * NP + l) + 1] = s;

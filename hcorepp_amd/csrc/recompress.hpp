@@ -315,16 +315,21 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, i
 
 // ---------------------------------------------------------------------------
 // The same block-reflector application B[:, c0:c1] <- (I - P op(T) P^T) B for 16-column FP64
-// panels, with both products on the matrix cores (v_mfma_f64_16x16x4f64).  Each wave owns whole
-// 16-column blocks of B, so the reduction over the rows happens inside the matrix core and no
-// barrier or cross-wave exchange is needed:
-//   W  = P^T B_blk    (16 x 16, K = rows: rows/4 MFMA steps; A = P^T from LDS, B from memory)
-//   W' = op(T) W      (VALU: W's columns gathered with shuffles, T from LDS)
-//   B_blk -= P W'     (per 16 rows: 4 MFMA steps of K = 4; A = P from LDS, B = W')
+// panels, with all three products on the matrix cores (v_mfma_f64_16x16x4f64).  Each wave owns
+// whole 16-column blocks of B, so the reduction over the rows happens inside the matrix core and
+// no barrier or cross-wave exchange is needed:
+//   W  = P^T B_blk    (16 x 16, K = rows: rows/4 MFMA steps in two interleaved chains)
+//   W' = op(T) W      (4 MFMA steps: W's accumulator is already the B operand layout)
+//   B_blk -= P W'     (per 16 rows: 4 MFMA steps of K = 4)
 // MFMA lane layout (f64 16x16x4): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
 // D[4 v + l / 16][l % 16] in element v of lane l (row groups of one, unlike the FP32 16x16x4
-// instruction's groups of four).  P: panel in LDS (rows x 16, ld pld, unit lower); Tm: 16 x 16
-// upper (LDS).
+// instruction's groups of four).  The reduction over rows may visit them in any order, and the
+// output rows of the update may be any permutation of the 16 rows of a group, so both products
+// are arranged for memory: in W, lane (j, k) supplies rows r0 + 4 k .. r0 + 4 k + 3 of column j
+// in steps 0..3; in the update, MFMA output row i is row r0 + 4 (i % 4) + i / 4, which puts
+// rows r0 + 4 k .. r0 + 4 k + 3 of column j in lane (j, k).  Each lane thus reads and writes 4
+// consecutive elements of its column per group.  P: panel in LDS (rows x 16, ld pld, unit
+// lower); Tm: 16 x 16 upper (LDS, zero below the diagonal).
 // ---------------------------------------------------------------------------
 typedef double hcamd_doublex4 __attribute__((ext_vector_type(4)));
 
@@ -332,51 +337,73 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
                                                int64_t ldb, int c0, int c1, bool trans) {
     const int l = lane_id(), i16 = l & 15, k4 = l >> 4;
     const int nblk = (c1 - c0 + 15) / 16;
+    // A operand of W' = op(T) W, step s: op(T)[i16][4 s + k4]
+    double ta[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = 4 * s + k4;
+        ta[s] = trans ? (k <= i16 ? Tm[k + 16 * i16] : 0.0) : (i16 <= k ? Tm[i16 + 16 * k] : 0.0);
+    }
+    const int r32 = rows & ~31;
+    const int prow = 4 * (i16 & 3) + (i16 >> 2);  // row of MFMA output row i16 within its group
     for (int blk = wave_id(); blk < nblk; blk += NWAVES) {
         const int col = c0 + 16 * blk + i16;
         const bool cok = col < c1;
+        // lanes past c1 read column c0 (in range) and discard it
         double *bcol = B + (int64_t) (cok ? col : c0) * ldb;
-        hcamd_doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < rows; k0 += 4) {
-            const int r = k0 + k4;
-            const double a = r < rows ? P[r + i16 * pld] : 0.0;
-            const double b = (r < rows && cok) ? bcol[r] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-        }
-        // acc[v] = W[4 v + k4][i16].  Gather column i16 of W: W[p][i16] is element p / 4 of lane
-        // i16 + 16 (p % 4).
-        double wc[16];
+        hcamd_doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+        int r0 = 0;
+        for (; r0 < r32; r0 += 32) {
+            const int ra = r0 + 4 * k4, rb = ra + 16;
+            double b0[4], b1[4], a0[4], a1[4];
 #pragma unroll
-        for (int p = 0; p < 16; ++p) wc[p] = __shfl(acc[p >> 2], i16 + 16 * (p & 3), WAVE);
-        // bop[s] = W'[4 s + k4][i16], the B operand of step s of the update
-        double bop[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int i = 4 * s + k4;
-            double t = 0.0;
-            if (trans) {
-#pragma unroll
-                for (int p = 0; p < 16; ++p) t += (p <= i) ? Tm[p + 16 * i] * wc[p] : 0.0;
-            } else {
-#pragma unroll
-                for (int p = 0; p < 16; ++p) t += (p >= i) ? Tm[i + 16 * p] * wc[p] : 0.0;
+            for (int s = 0; s < 4; ++s) {
+                b0[s] = bcol[ra + s];
+                b1[s] = bcol[rb + s];
             }
-            bop[s] = t;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                a0[s] = P[ra + s + i16 * pld];
+                a1[s] = P[rb + s + i16 * pld];
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s], cok ? b0[s] : 0.0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[s], cok ? b1[s] : 0.0, acc1, 0, 0, 0);
+            }
         }
-        for (int r0 = 0; r0 < rows; r0 += 16) {
+        for (; r0 < rows; r0 += 16) {
+            const int ra = r0 + 4 * k4;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int r = ra + s;
+                const double a = r < rows ? P[r + i16 * pld] : 0.0;
+                const double b = (r < rows && cok) ? bcol[r] : 0.0;
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
+            }
+        }
+        // acc[v] = W[4 v + k4][i16]: element s is the B operand of step s of W' = op(T) W
+        const hcamd_doublex4 acc = acc0 + acc1;
+        hcamd_doublex4 wp = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) wp = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[s], acc[s], wp, 0, 0, 0);
+        // wp[s] = W'[4 s + k4][i16], the B operand of step s of the update
+#pragma unroll 2
+        for (int g = 0; g < rows; g += 16) {
+            const int pa = g + prow, rq = g + 4 * k4;
+            double old[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) old[v] = (rq + v < rows) ? bcol[rq + v] : 0.0;
             hcamd_doublex4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const int r = r0 + i16;
-                const double a = r < rows ? P[r + (4 * s + k4) * pld] : 0.0;
-                d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bop[s], d, 0, 0, 0);
+                const double a = pa < rows ? P[pa + (4 * s + k4) * pld] : 0.0;
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wp[s], d, 0, 0, 0);
             }
             if (cok) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = r0 + 4 * v + k4;
-                    if (r < rows) bcol[r] -= d[v];
-                }
+                for (int v = 0; v < 4; ++v)
+                    if (rq + v < rows) bcol[rq + v] = old[v] - d[v];
             }
         }
     }

@@ -7,6 +7,10 @@ post-truncation ranks, trains the MLP with the library's FP32 kernels, and write
 to hcorepp_amd/rank_mlp.npz (with validation statistics).
 
     python -m hcorepp_amd.train_rank_model [--batches 8] [--per-batch 4096] [--epochs 120]
+
+Collection can be split over several processes: `--collect-only FILE --first-batch K` writes the
+features and ranks of batches K .. K+batches-1 to FILE (no training), and `--from FILE [FILE ...]`
+trains on collected files (the last batch of the last file is the validation set).
 """
 from __future__ import annotations
 
@@ -32,8 +36,8 @@ def log(*a):
 def collect(ctx, n, seed):
     """Features and exact ranks of one batch of n updates."""
     import hcorepp_amd as H
-    from hcorepp_amd.workloads import batch_ccc_pool
-    ups = batch_ccc_pool(n, NB, seed, LO, HI, DECAY)
+    from hcorepp_amd.workloads import batch_ccc
+    ups = batch_ccc(n, NB, seed, LO, HI, DECAY, threads=min(16, os.cpu_count() or 8))
     As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
     Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
     arena = H.TileArena(n * H.TileArena.compressed_bytes(NB, NB, MAX_RANK))
@@ -54,18 +58,37 @@ def main():
     ap.add_argument("--per-batch", type=int, default=4096)
     ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--first-batch", type=int, default=0, help="index of the first batch (seed offset)")
+    ap.add_argument("--collect-only", default=None, help="write features and ranks here, do not train")
+    ap.add_argument("--from", dest="sources", nargs="+", default=None, help="train on collected files")
     args = ap.parse_args()
     import hcorepp_amd as H
     from hcorepp_amd.learned import RankPredictor
     ctx = H.RunContext(0)
     Xs, ys = [], []
     t0 = time.time()
-    for k in range(args.batches + 1):  # the last batch is held out for validation
-        X, r = collect(ctx, args.per_batch, TRAIN_SEED0 + k)
-        Xs.append(X)
-        ys.append(np.log2(r))
-        log(f"[train] batch {k}: {args.per_batch} recompressions, ranks {int(r.min())}-{int(r.max())} "
-            f"({time.time() - t0:.1f}s)")
+    if args.sources:
+        for f in args.sources:
+            with np.load(f) as z:
+                for k in range(int(z["batches"])):
+                    Xs.append(z[f"x{k}"])
+                    ys.append(z[f"y{k}"])
+        log(f"[train] {len(Xs)} collected batches from {len(args.sources)} file(s)")
+    else:
+        # the last batch is held out for validation (collection: simply the last one written)
+        nb_ = args.batches if args.collect_only else args.batches + 1
+        for k in range(nb_):
+            X, r = collect(ctx, args.per_batch, TRAIN_SEED0 + args.first_batch + k)
+            Xs.append(X)
+            ys.append(np.log2(r))
+            log(f"[train] batch {args.first_batch + k}: {args.per_batch} recompressions, ranks "
+                f"{int(r.min())}-{int(r.max())} ({time.time() - t0:.1f}s)")
+        if args.collect_only:
+            np.savez(args.collect_only, batches=len(Xs), first_batch=args.first_batch,
+                     **{f"x{k}": x for k, x in enumerate(Xs)}, **{f"y{k}": y for k, y in enumerate(ys)})
+            log(f"[train] wrote {args.collect_only}")
+            ctx.close()
+            return
     Xtr, ytr = np.concatenate(Xs[:-1]), np.concatenate(ys[:-1])
     Xva, yva = Xs[-1], ys[-1]
     model = RankPredictor(ctx, seed=1)

@@ -5,6 +5,10 @@
   reference's OpenMP example (examples/matrix_multiplication/omp_main.cpp:112-126), 64 updates
   (24 CCC, 12 DCC, 12 CDC, 12 CCD, 4 DDD) through the plain HIP path, each compared with the oracle
   run on the same input (the device's C before the update);
+* one C tile of the 16 x 16-tile, nb = 1024 TLR GEMM of exp(-r/0.2) over 16384 points in the unit
+  cube (seed 20261203), acc 1e-10, maxRank 341: C(1, 9) through all 16 updates (14 CCC, one DCC,
+  one CDC), each compared with the oracle run on the device's C before the update, and the final
+  tile against the dense FP64 sum;
 * the TLR Cholesky of an 8 x 8-tile, nb = 512 matrix exp(-r/0.1) + 1e-2 I over 4096 points in the
   unit cube, acc 1e-8, against the oracle's tile sequence (oracle/cholesky.py);
 * an 8192-update, nb = 512 batch of the benchmark's distribution with learned core sizing: every
@@ -81,6 +85,42 @@ def test_tlr_gemm_4x4_nb256_matches_oracle(ctx):
     assert ties == 0
     # the rank cap nb/3 = 85 bounds the accuracy of the most coupled tiles
     assert max(final_err) < 1e-7
+
+
+def test_tlr_gemm_nb1024_c_tile_matches_oracle(ctx):
+    H, O = _mods()
+    from hcorepp_amd.workloads import TLRMatrix, prefetch_tiles
+    acc, i, j = 1e-10, 1, 9
+    M = TLRMatrix(16384, 1024, 3, 0.2, acc, 20261203)
+    T = M.T
+    prefetch_tiles(M, sorted({(i, k) for k in range(T)} | {(k, j) for k in range(T)}), 16)
+    params, oparams = H.CompressionParameters(acc), O.Params(acc)
+    tc = M.tile(i, j)
+    C = H.CompressedTile(tc[1], tc[2], max_rank=tc[3])
+    ref = tc[1] @ tc[2]
+    kinds, ranks = {}, []
+    for k in range(T):
+        ta, tb = M.tile(i, k), M.tile(k, j)
+        kind = ta[0] + tb[0] + "C"
+        kinds[kind] = kinds.get(kind, 0) + 1
+        print(f"[case] nb=1024 tile ({i},{j}) k={k} {kind}", flush=True)
+        A = H.DenseTile(ta[1]) if ta[0] == "D" else H.CompressedTile(ta[1], ta[2], max_rank=ta[3])
+        B = H.DenseTile(tb[1]) if tb[0] == "D" else H.CompressedTile(tb[1], tb[2], max_rank=tb[3])
+        oA = O.Tile.dense(ta[1].copy(order="F")) if ta[0] == "D" else O.Tile.compressed(ta[1], ta[2])
+        oB = O.Tile.dense(tb[1].copy(order="F")) if tb[0] == "D" else O.Tile.compressed(tb[1], tb[2])
+        u, v = C.factors()
+        oC = O.Tile.compressed(u, v, max_rank=tc[3])
+        _, so = O.hcore_gemm(1.0, oA, oB, 1.0, oC, oparams, want_sigma=True)
+        H.HCore.gemm(1.0, A, B, 1.0, C, ctx, params)
+        assert rank_tie(so, oC.rank, C.rank, acc), (k, oC.rank, C.rank)
+        if C.rank == oC.rank:
+            assert rel_fro(C.to_dense(), oC.to_dense()) < 1e-10, k
+        ranks.append(C.rank)
+        ref += (ta[1] if ta[0] == "D" else ta[1] @ ta[2]) @ (tb[1] if tb[0] == "D" else tb[1] @ tb[2])
+    assert kinds == {"CCC": 14, "DCC": 1, "CDC": 1}, kinds
+    err = rel_fro(C.to_dense(), ref)
+    print(f"nb=1024 C({i},{j}): ranks {min(ranks)}-{max(ranks)}, error vs dense FP64 {err:.2e}")
+    assert err < 1e-8
 
 
 def test_tlr_cholesky_8x8_nb512_3d_matches_oracle(ctx):

@@ -123,8 +123,10 @@ def test_learned_core_sizing_matches_exact_and_falls_back(ctx):
 def test_learned_sizing_gain_on_benchmark_distribution(ctx):
     """The committed rank model (hcorepp_amd/rank_mlp.npz) on 1024 updates of the benchmark's
     distribution (nb=512, ranks log-uniform [8, 192]): same ranks as the exact form, results within
-    1e-8, and narrower cores: the Jacobi's work (sum of cubed core widths) drops.  Pass times of
-    both forms are printed."""
+    1e-8, narrower cores (the sum of cubed core widths drops), and a shorter launch: each form is
+    timed as the median of 3 launches, the C tiles restored from a device copy before each.  The
+    learned launch must be faster than the exact one (measured 1.06x on the 4096-update benchmark
+    batch, short of 1.2x; the tolerance rule on the pivoted R is faster still)."""
     import os
     import time
     H, Lm, M = _mods()
@@ -138,23 +140,35 @@ def test_learned_sizing_gain_on_benchmark_distribution(ctx):
     bw.close()
     forms = {}
     for name in ("exact", "learned"):
-        b, Cs = _batch(H, ctx, ups, acc=1e-8, mr=384)
+        As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
+        Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
+        arena = H.TileArena(len(ups) * H.TileArena.compressed_bytes(512, 512, 384))
+        Cs = [H.CompressedTile(u["cu"], u["cv"], max_rank=384, arena=arena) for u in ups]
+        c0 = H.DeviceBuffer(arena.off)
+        c0.copy_from(arena.buf, arena.off)
+        r0 = np.array([c.rank for c in Cs], dtype=np.int64)
+        b = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, H.CompressionParameters(1e-8), H.PREC_FP64)
         if name == "learned":
             b.set_rank_model(model.bf16(), tol=0.1, margin=40)
-        ctx.sync()
-        t = time.perf_counter()
-        b.run()
-        ctx.sync()
-        dt = time.perf_counter() - t
-        forms[name] = (b, Cs, b.core_widths(), dt)
-    (be, Ce, ke, te), (bl, Cl, kl, tl) = forms["exact"], forms["learned"]
-    print(f"exact {te * 1e3:.1f} ms, learned {tl * 1e3:.1f} ms; sum kt^2 {np.sum(ke ** 2.0):.3e} -> "
-          f"{np.sum(kl ** 2.0):.3e}; fallbacks {bl.learned_stats()[0]}")
+        times = []
+        for _ in range(3):
+            H.lib().hcorepp_memcpy_async(ctx.h, arena.buf.ptr, c0.ptr, arena.off)
+            b.set_ranks(r0)
+            ctx.sync()
+            t = time.perf_counter()
+            b.run()
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+        forms[name] = (b, Cs, b.core_widths(), float(np.median(times)), (As, Bs, arena, c0))
+    (be, Ce, ke, te, _), (bl, Cl, kl, tl, _) = forms["exact"], forms["learned"]
+    print(f"exact {te * 1e3:.1f} ms, learned {tl * 1e3:.1f} ms ({te / tl:.3f}x); sum kt^3 "
+          f"{np.sum(ke ** 3.0):.3e} -> {np.sum(kl ** 3.0):.3e}; fallbacks {bl.learned_stats()[0]}")
     re, rl = be.sync_ranks_to_tiles(), bl.sync_ranks_to_tiles()
     assert np.array_equal(re, rl)
     for i in range(0, len(ups), 16):
         assert rel_fro(Cl[i].to_dense(), Ce[i].to_dense()) < 1e-8
     # the Jacobi on a kt x kt core (after the core QR) costs ~ kt^3 per sweep
     assert np.sum(kl ** 3.0) < 0.97 * np.sum(ke ** 3.0)
+    assert tl < te, (te, tl)
     be.close()
     bl.close()
