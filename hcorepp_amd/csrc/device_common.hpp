@@ -239,6 +239,20 @@ __device__ __forceinline__ int wave_multi_sum_index(int l) {
     return l >> (6 - log2_const<NV>());
 }
 
+// dst[e] = src[e] for e in [0, n) by the whole workgroup, four loads in flight per thread.
+template <typename T>
+__device__ __forceinline__ void wg_copy(T *dst, const T *src, int64_t n) {
+    int64_t e = threadIdx.x;
+    for (; e + 3 * NT < n; e += 4 * NT) {
+        const T a = src[e], b = src[e + NT], c = src[e + 2 * NT], d = src[e + 3 * NT];
+        dst[e] = a;
+        dst[e + NT] = b;
+        dst[e + 2 * NT] = c;
+        dst[e + 3 * NT] = d;
+    }
+    for (; e < n; e += NT) dst[e] = src[e];
+}
+
 template <typename T>
 __device__ __forceinline__ T dsign(T a, T b) {
     return b >= T(0) ? (a >= T(0) ? a : -a) : (a >= T(0) ? -a : a);

@@ -58,28 +58,6 @@ __device__ __forceinline__ constexpr int multi_sum_src_lane(int j) {
     return j << (6 - log2_const<NV>());
 }
 
-template <int TW, bool FULL>
-__host__ __device__ __forceinline__ constexpr int jpair_p(int rd, int k) {
-    if (FULL) {
-        const int a = (k == 0) ? TW - 1 : (rd + k) % (TW - 1);
-        const int b = (k == 0) ? rd : (rd - k + TW - 1) % (TW - 1);
-        return a < b ? a : b;
-    } else {
-        return k;
-    }
-}
-
-template <int TW, bool FULL>
-__host__ __device__ __forceinline__ constexpr int jpair_q(int rd, int k) {
-    if (FULL) {
-        const int a = (k == 0) ? TW - 1 : (rd + k) % (TW - 1);
-        const int b = (k == 0) ? rd : (rd - k + TW - 1) % (TW - 1);
-        return a < b ? b : a;
-    } else {
-        return TW / 2 + (k + rd) % (TW / 2);
-    }
-}
-
 // Rotation (c, s) that annihilates apq for the Gram entries (app, aqq, apq) of a column pair,
 // or (1, 0) when the pair is already orthogonal to tolerance or involves a numerically zero
 // column.  t = tan(theta) is the smaller root of t^2 + 2 zeta t - 1 = 0 with
@@ -115,8 +93,27 @@ __host__ __device__ constexpr int vrows() {
 // reduce-scattered across the wave with permlane/DPP exchanges; lane k < TW/2 gathers the three
 // sums of pair k, computes its rotation, and every lane reads the rotations back with readlane.
 // The rotations are applied to the J tile y and to the matching V columns vt (none for RPLV 0).
+// The pairs sit in fixed register slots so that the round loop can stay rolled (unrolled rounds
+// of a 16-column tile are tens of KB of code, more than the instruction cache holds next to the
+// rest of the kernel): pair k is slots (k, TW - 1 - k) in a full tile (all pairs of its columns) and
+// (k, TW/2 + k) in a cross tile (pairs between its two blocks); between rounds the caller moves
+// the columns to the next slots (jacobi_rotate_slots), the circle method of round-robin pairing.
+// The rotation of a pair does not depend on which of its columns is named first.
+template <typename T, int TW, bool FULL>
+__host__ __device__ __forceinline__ constexpr int jslot_q(int k) {
+    return FULL ? TW - 1 - k : TW / 2 + k;
+}
+
+// Tile column held by slot `slot` in round rd of the rolled rounds (jacobi_rotate_slots moves the
+// content of slot i + 1 to slot i, wrapping within the rotated range).
+template <int TW, bool FULL>
+__host__ __device__ __forceinline__ constexpr int jslot_col(int rd, int slot) {
+    if (FULL) return slot == TW - 1 ? TW - 1 : (slot + rd) % (TW - 1);
+    return slot < TW / 2 ? slot : TW / 2 + (slot - TW / 2 + rd) % (TW / 2);
+}
+
 template <typename T, int RPL, int RPLV, int TW, bool FULL>
-__device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], int rd, T tol, T floor2) {
+__device__ __forceinline__ int jacobi_round_fixed(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], T tol, T floor2) {
     constexpr int NP = TW / 2;
     constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
     const int l = lane_id();
@@ -125,10 +122,10 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>
     for (int k = 0; k < NV; ++k) part[k] = T(0);
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+        const int q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
-            const T a = y[t][p], b = y[t][q];
+            const T a = y[t][k], b = y[t][q];
             part[3 * k] += a * a;
             part[3 * k + 1] += b * b;
             part[3 * k + 2] += a * b;
@@ -149,23 +146,40 @@ __device__ __forceinline__ int jacobi_round(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>
         const T sk = read_lane(s, k);
         if (sk != T(0)) {
             const T ck = read_lane(c, k);
-            const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+            const int q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
-                const T a = y[r][p], b = y[r][q];
-                y[r][p] = ck * a - sk * b;
+                const T a = y[r][k], b = y[r][q];
+                y[r][k] = ck * a - sk * b;
                 y[r][q] = sk * a + ck * b;
             }
 #pragma unroll
             for (int r = 0; r < RPLV; ++r) {
-                const T va = vt[r][p], vb = vt[r][q];
-                vt[r][p] = ck * va - sk * vb;
+                const T va = vt[r][k], vb = vt[r][q];
+                vt[r][k] = ck * va - sk * vb;
                 vt[r][q] = sk * va + ck * vb;
             }
             any |= JROT_APPLIED | (((lgm >> k) & 1ull) ? JROT_LARGE : 0);
         }
     }
     return any;
+}
+
+// Columns to their slots for the next round: a full tile keeps slot TW-1 and rotates slots
+// 0..TW-2 down by one, so that round rd pairs column TW-1 with column rd and columns rd + k with
+// rd - k (mod TW - 1): after TW - 1 rounds every pair has met once and the columns are back in
+// place.  A cross tile rotates the second block's slots TW/2..TW-1 down by one (column k of the
+// first block meets column (k + rd) mod TW/2 of the second; back in place after TW/2 rounds).
+template <typename T, int ROWS, int TW, bool FULL>
+__device__ __forceinline__ void jacobi_rotate_slots(T (&y)[ROWS][TW]) {
+    constexpr int FIRST = FULL ? 0 : TW / 2, LAST = FULL ? TW - 2 : TW - 1;
+#pragma unroll
+    for (int t = 0; t < ROWS; ++t) {
+        const T first = y[t][FIRST];
+#pragma unroll
+        for (int c = FIRST; c < LAST; ++c) y[t][c] = y[t][c + 1];
+        y[t][LAST] = first;
+    }
 }
 
 // Process one tile (block pair) with one wave.  Columns [cI, cI+TW/2) and [cJ, cJ+TW/2)
@@ -194,11 +208,19 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
     }
     int any = 0;
     if (full) {
-#pragma unroll
-        for (int rd = 0; rd < TW - 1; ++rd) any |= jacobi_round<T, RPL, RPLV, TW, true>(y, vt, rd, tol, floor2);
+#pragma unroll 1
+        for (int rd = 0; rd < TW - 1; ++rd) {
+            any |= jacobi_round_fixed<T, RPL, RPLV, TW, true>(y, vt, tol, floor2);
+            jacobi_rotate_slots<T, RPL, TW, true>(y);
+            if constexpr (RPLV > 0) jacobi_rotate_slots<T, RPLV, TW, true>(vt);
+        }
     } else {
-#pragma unroll
-        for (int rd = 0; rd < TW / 2; ++rd) any |= jacobi_round<T, RPL, RPLV, TW, false>(y, vt, rd, tol, floor2);
+#pragma unroll 1
+        for (int rd = 0; rd < TW / 2; ++rd) {
+            any |= jacobi_round_fixed<T, RPL, RPLV, TW, false>(y, vt, tol, floor2);
+            jacobi_rotate_slots<T, RPL, TW, false>(y);
+            if constexpr (RPLV > 0) jacobi_rotate_slots<T, RPLV, TW, false>(vt);
+        }
     }
     if (any & JROT_APPLIED) {
 #pragma unroll
@@ -293,8 +315,7 @@ __device__ __forceinline__ int wg_block_jacobi(T *J, int rows, int ncols, int64_
 // allow mean fewer passes over J and V per sweep (traffic ~ ncols / (TW / 2) passes).
 // ---------------------------------------------------------------------------
 template <typename T, int RPL, int RPLV, int TW, bool FULL>
-__device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], int rd, T tol, T floor2, T *red,
-                                          int parity) {
+__device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], T tol, T floor2, T *red, int parity) {
     constexpr int NP = TW / 2;
     constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
     const int l = lane_id(), w = wave_id();
@@ -303,7 +324,7 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
     for (int k = 0; k < NV; ++k) part[k] = T(0);
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+        const int p = k, q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const T a = y[t][p], b = y[t][q];
@@ -336,7 +357,7 @@ __device__ __forceinline__ int coop_round(T (&y)[RPL][TW], T (&vt)[RPLV][TW], in
         const T sk = read_lane(s, k);
         if (sk != T(0)) {
             const T ck = read_lane(c, k);
-            const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+            const int p = k, q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 const T a = y[r][p], b = y[r][q];
@@ -383,15 +404,19 @@ __device__ int coop_jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, in
     }
     int any = 0;
     if (full) {
-#pragma unroll
+#pragma unroll 1
         for (int rd = 0; rd < TW - 1; ++rd) {
-            any |= coop_round<T, RPL, RPLV, TW, true>(y, vt, rd, tol, floor2, red, parity);
+            any |= coop_round<T, RPL, RPLV, TW, true>(y, vt, tol, floor2, red, parity);
+            jacobi_rotate_slots<T, RPL, TW, true>(y);
+            if constexpr (RPLV > 0) jacobi_rotate_slots<T, RPLV, TW, true>(vt);
             parity ^= 1;
         }
     } else {
-#pragma unroll
+#pragma unroll 1
         for (int rd = 0; rd < TW / 2; ++rd) {
-            any |= coop_round<T, RPL, RPLV, TW, false>(y, vt, rd, tol, floor2, red, parity);
+            any |= coop_round<T, RPL, RPLV, TW, false>(y, vt, tol, floor2, red, parity);
+            jacobi_rotate_slots<T, RPL, TW, false>(y);
+            if constexpr (RPLV > 0) jacobi_rotate_slots<T, RPLV, TW, false>(vt);
             parity ^= 1;
         }
     }
@@ -437,7 +462,7 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
     for (int k = 0; k < NV; ++k) part[k] = T(0);
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+        const int p = k, q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const T a = y[t][p], b = y[t][q];
@@ -474,7 +499,7 @@ __device__ __forceinline__ int coop_round_rec(T (&y)[RPL][TW], int rd, T tol, T 
         const T sk = read_lane(s, k);
         if (sk != T(0)) {
             const T ck = read_lane(c, k);
-            const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+            const int p = k, q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 const T a = y[r][p], b = y[r][q];
@@ -506,7 +531,7 @@ __device__ __forceinline__ void coop_apply_v(T *V, int ncols, int64_t ldv, int c
             for (int k = 0; k < NP; ++k) {
                 const T sk = cs[2 * (rd * NP + k) + 1];
                 const T ck = cs[2 * (rd * NP + k)];
-                const int p = jpair_p<TW, FULL>(rd, k), q = jpair_q<TW, FULL>(rd, k);
+                const int p = jslot_col<TW, FULL>(rd, k), q = jslot_col<TW, FULL>(rd, jslot_q<T, TW, FULL>(k));
                 const T a = v[p], b = v[q];
                 v[p] = ck * a - sk * b;
                 v[q] = sk * a + ck * b;
@@ -543,15 +568,17 @@ __device__ __forceinline__ int coop_jacobi_tile_rec(T *J, int rows, int ncols, i
     }
     int any = 0, parity = 0;
     if (full) {
-#pragma unroll
+#pragma unroll 1
         for (int rd = 0; rd < TW - 1; ++rd) {
             any |= coop_round_rec<T, RPL, TW, true>(y, rd, tol, floor2, red, parity, cs);
+            jacobi_rotate_slots<T, RPL, TW, true>(y);
             parity ^= 1;
         }
     } else {
-#pragma unroll
+#pragma unroll 1
         for (int rd = 0; rd < TW / 2; ++rd) {
             any |= coop_round_rec<T, RPL, TW, false>(y, rd, tol, floor2, red, parity, cs);
+            jacobi_rotate_slots<T, RPL, TW, false>(y);
             parity ^= 1;
         }
     }

@@ -122,7 +122,7 @@ __global__ void HCAMD_KERNEL k_qr_panel(RecompDesc<T> d, int which, int j0, T *v
                               q.pivot ? cn : nullptr);
     }
     const int rows = q.m - j0;
-    for (int e = threadIdx.x; e < panel_ld(rows) * PB; e += NT) vpan[e] = lds[e];
+    wg_copy(vpan, lds, (int64_t) panel_ld(rows) * PB);
 }
 
 template <typename T, int PB, int RPT>

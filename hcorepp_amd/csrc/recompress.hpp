@@ -480,12 +480,15 @@ __host__ __device__ inline int64_t qr_lds_elems(int64_t m, int pb) {
 // Window pivoting of a blocked QR at panel start j0 (column norms from `cn` when given): the jb trailing columns of largest
 // norm over rows [j0, m) are moved, in decreasing norm order, to positions [j0, j0 + jb)
 // by jb column swaps (whole columns, so the R rows above j0 move with them); piv records
-// the permutation.  Uses the panel buffer P as scratch (needs m * PB >= 2 * (ncols - j0) + 2 * PB).
+// the permutation.  Uses the panel buffer P as scratch (needs m * PB >= 2 * (ncols - j0) + 2 * PB)
+// and, for the move list, the 4 PB + 1 ints after the panel ((m | 1) * PB elements on), where the
+// panel's T factor goes later.
 template <typename T, int PB>
 __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, int32_t *piv, T *P,
                           const T *cn = nullptr) {
     const int nt = ncols - j0;
     const int l = lane_id();
+    int *mv = reinterpret_cast<int *>(P + (int64_t) (m | 1) * PB);
     T *nrm = P;
     int *at = reinterpret_cast<int *>(nrm + nt);  // at[p]: column (by starting position) now at p
     int *where = at + nt;                         // where[c]: current position of column c
@@ -525,6 +528,10 @@ __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, i
         if (pos < jb) sel[pos] = c;
     }
     __syncthreads();
+    // The jb swaps (t, where[sel[t]]) in turn define the permutation; afterwards at[q] is the
+    // position (relative to j0) whose column ends at q.  The positions that changed and their
+    // sources go to the move list mv (pairs destination, source; count in mv[4 PB]), so the
+    // columns move in one pass over the rows.
     if (threadIdx.x == 0) {
         for (int t = 0; t < jb; ++t) {
             const int c = sel[t], p = where[c];
@@ -540,21 +547,34 @@ __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, i
                 piv[j0 + p] = x;
             }
         }
-    }
-    __syncthreads();
-    for (int t = 0; t < jb; ++t) {
-        const int p = sel[PB + t];
-        if (p != t) {
-            T *ca = A + (int64_t) (j0 + t) * lda;
-            T *cb = A + (int64_t) (j0 + p) * lda;
-            for (int r = threadIdx.x; r < m; r += NT) {
-                const T x = ca[r];
-                ca[r] = cb[r];
-                cb[r] = x;
+        int nmv = 0;
+        for (int i = 0; i < 2 * jb; ++i) {
+            const int q = i < jb ? i : sel[PB + i - jb];
+            if (at[q] == q) continue;
+            bool seen = false;
+            for (int k = 0; k < nmv; ++k) seen |= mv[2 * k] == q;
+            if (!seen) {
+                mv[2 * nmv] = q;
+                mv[2 * nmv + 1] = at[q];
+                ++nmv;
             }
         }
-        __syncthreads();
+        mv[4 * PB] = nmv;
     }
+    __syncthreads();
+    const int nmv = mv[4 * PB];
+    if (nmv > 0) {
+        for (int r = threadIdx.x; r < m; r += NT) {
+            T v[2 * PB];
+#pragma unroll
+            for (int i = 0; i < 2 * PB; ++i)
+                if (i < nmv) v[i] = A[r + (int64_t) (j0 + mv[2 * i + 1]) * lda];
+#pragma unroll
+            for (int i = 0; i < 2 * PB; ++i)
+                if (i < nmv) A[r + (int64_t) (j0 + mv[2 * i]) * lda] = v[i];
+        }
+    }
+    __syncthreads();
     return trail2;
 }
 
@@ -592,20 +612,43 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
     HCAMD_SUB_BEGIN(t_piv);
     if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P, cn);
     HCAMD_SUB_END(t_piv, hdr, 5);
-    // (column-outer loops over the compile-time panel width: no per-element index division)
+    // one row per thread and iteration, the PB loads of a row in flight together
+    for (int r = threadIdx.x; r < rows; r += NT) {
+        T v[PB];
 #pragma unroll
-    for (int c = 0; c < PB; ++c)
-        for (int r = threadIdx.x; r < rows; r += NT)
-            P[r + c * pld] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
+        for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
+#pragma unroll
+        for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
+    }
     __syncthreads();
+    // Householder steps with one pass over the rows and two barriers per column: column jj's
+    // partial products with itself and with the later panel columns are reduced together, which
+    // gives its norm and the dot products of the (still unscaled) reflector with those columns.
+    // Every lane then forms the reflector and the update coefficients itself: lane k holds the
+    // sums of value k, and readlane spreads them.  Row jj is copied to `xrow` before the barrier
+    // because its owner updates it in place.
+    T *xrow = red + NWAVES * PB;
+    const int kl = l & (PB - 1);
     for (int jj = 0; jj < jb; ++jj) {
-        T part = T(0);
+        T acc[PB];
+#pragma unroll
+        for (int k = 0; k < PB; ++k) acc[k] = T(0);
         for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-            const T v = P[r + jj * pld];
-            part += v * v;
+            const T x = P[r + jj * pld];
+#pragma unroll
+            for (int k = 0; k < PB; ++k)
+                if (k >= jj && k < jb) acc[k] += x * P[r + k * pld];
         }
-        const T xnorm2 = wg_sum(part, red);
-        const T alpha = P[jj + jj * pld];
+        const T s = wave_multi_sum_fast<T, PB>(acc);
+        const int idx = wave_multi_sum_index<PB>(l);
+        if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
+        if (threadIdx.x < PB) xrow[threadIdx.x] = P[jj + threadIdx.x * pld];
+        __syncthreads();
+        T sk = T(0);
+#pragma unroll
+        for (int wv = 0; wv < NWAVES; ++wv) sk += red[wv * PB + kl];
+        const T xk = xrow[kl];
+        const T xnorm2 = read_lane(sk, jj), alpha = read_lane(xk, jj);
         T tau_j, beta, scal;
         if (xnorm2 == T(0)) {
             tau_j = T(0);
@@ -616,54 +659,39 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
             tau_j = (beta - alpha) / beta;
             scal = T(1) / (alpha - beta);
         }
-        __syncthreads();
-        if (tau_j != T(0)) {
-            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) P[r + jj * pld] *= scal;
-        }
+        // w_k = x_k[jj] + sum_{r > jj} v_r x_k[r] with v_r = scal x_jj[r]; update x_k -= tau w_k v
+        const T gk = (kl > jj && kl < jb) ? tau_j * (xk + scal * sk) : T(0);
+        T g[PB];
+#pragma unroll
+        for (int k = 0; k < PB; ++k) g[k] = read_lane(gk, k);
         if (threadIdx.x == 0) {
             P[jj + jj * pld] = beta;
             tau[j0 + jj] = tau_j;
+#pragma unroll
+            for (int k = 0; k < PB; ++k)
+                if (k > jj && k < jb) P[jj + k * pld] -= g[k];
+        }
+        if (tau_j != T(0)) {
+            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
+                const T v = scal * P[r + jj * pld];
+                P[r + jj * pld] = v;
+#pragma unroll
+                for (int k = 0; k < PB; ++k)
+                    if (k > jj && k < jb) P[r + k * pld] -= g[k] * v;
+            }
         }
         __syncthreads();
-        if (tau_j != T(0) && jj + 1 < jb) {
-            // w_k = x_k[jj] + sum_{r>jj} v_r x_k[r] for panel columns k in (jj, jb)
-            T acc[PB];
-#pragma unroll
-            for (int k = 0; k < PB; ++k) acc[k] = T(0);
-            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-                const T v = P[r + jj * pld];
-#pragma unroll
-                for (int k = 0; k < PB; ++k)
-                    if (k > jj && k < jb) acc[k] += v * P[r + k * pld];
-            }
-            const T s = wave_multi_sum_fast<T, PB>(acc);
-            const int idx = wave_multi_sum_index<PB>(l);
-            if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
-            __syncthreads();
-            if (threadIdx.x < PB) {
-                const int k = threadIdx.x;
-                if (k > jj && k < jb) {
-                    T wsum = P[jj + k * pld];
-                    for (int wv = 0; wv < NWAVES; ++wv) wsum += red[wv * PB + k];
-                    gram[k] = tau_j * wsum;
-                }
-            }
-            __syncthreads();
-            for (int r = jj + threadIdx.x; r < rows; r += NT) {
-                const T v = (r == jj) ? T(1) : P[r + jj * pld];
-#pragma unroll
-                for (int k = 0; k < PB; ++k)
-                    if (k > jj && k < jb) P[r + k * pld] -= gram[k] * v;
-            }
-            __syncthreads();
-        }
     }
     HCAMD_SUB_END(t_pan, hdr, 0);
-    // write the factored panel back (R on/above the diagonal, reflectors below)
+    // write the factored panel back (R on/above the diagonal, reflectors below), a row per thread
+    for (int r = threadIdx.x; r < rows; r += NT) {
+        T v[PB];
 #pragma unroll
-    for (int c = 0; c < PB; ++c)
-        if (c < jb)
-            for (int r = threadIdx.x; r < rows; r += NT) A[(j0 + r) + (int64_t) (j0 + c) * lda] = P[r + c * pld];
+        for (int c = 0; c < PB; ++c) v[c] = P[r + c * pld];
+#pragma unroll
+        for (int c = 0; c < PB; ++c)
+            if (c < jb) A[(j0 + r) + (int64_t) (j0 + c) * lda] = v[c];
+    }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < PB; ++c)
@@ -715,16 +743,15 @@ __device__ void wg_apply_q(int m, int nref, const T *A, int64_t lda, const T *ts
         const int jb = (nref - j0) < PB ? (nref - j0) : PB;
         const int rows = m - j0;
         const int pld = panel_ld(rows);
+        // unit-lower reflectors: one row per thread and iteration, its PB loads in flight together
+        for (int r = threadIdx.x; r < rows; r += NT) {
+            T v[PB];
 #pragma unroll
-        for (int c = 0; c < PB; ++c)
-            for (int r = threadIdx.x; r < rows; r += NT) {
-                T v = T(0);
-                if (c < jb) {
-                    if (r == c) v = T(1);
-                    else if (r > c) v = A[(j0 + r) + (int64_t) (j0 + c) * lda];
-                }
-                P[r + c * pld] = v;
-            }
+            for (int c = 0; c < PB; ++c)
+                v[c] = (c < jb && r > c) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(c < jb && r == c ? 1 : 0);
+#pragma unroll
+            for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
+        }
         for (int e = threadIdx.x; e < PB * PB; e += NT) Tm[e] = tsave[(int64_t) p * PB * PB + e];
         __syncthreads();
         apply_wy_dispatch<T, PB>(P, pld, Tm, rows, B + j0, ldb, c0, c1, false, red);
@@ -1111,7 +1138,7 @@ __device__ int run_jacobi_nov(T *J, int rows, int ncols, T *lds) {
     const bool resident = ne <= nov_lds_cap<T>();
     T *Jl = lds + NOV_GBUF;
     if (resident) {
-        for (int64_t e = threadIdx.x; e < ne; e += NT) Jl[e] = J[e];
+        wg_copy(Jl, J, ne);
         __syncthreads();
     }
     int sweeps;
@@ -1124,7 +1151,7 @@ __device__ int run_jacobi_nov(T *J, int rows, int ncols, T *lds) {
     else sweeps = jacobi_nov_shape<T, 12, 4>(J, rows, ncols, lds, resident);
     if (resident) {
         __syncthreads();
-        for (int64_t e = threadIdx.x; e < ne; e += NT) J[e] = Jl[e];
+        wg_copy(J, Jl, ne);
         __syncthreads();
     }
     return sweeps;

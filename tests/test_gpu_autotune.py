@@ -1,6 +1,6 @@
 """The batched kernel's launch shape (Householder panel width, workgroups per CU): the default
 comes from the compiled-in autotune table for the batch's class, and every valid panel width
-gives the same ranks and the same factors to roundoff as the others."""
+gives the same ranks and the same factors as the others (to the batched Jacobi's stopping level)."""
 import numpy as np
 import pytest
 
@@ -35,7 +35,8 @@ def test_default_shape_from_table(ctx):
 @pytest.mark.parametrize("nb,lo,hi", [(256, 8, 96), (512, 8, 192)])
 def test_panel_widths_agree(ctx, nb, lo, hi):
     """Panel widths 4, 8 and 16 (and both residencies) give the same ranks, and factors equal to
-    roundoff, on one batch; a captured graph is dropped by the change and captured again."""
+    the Jacobi's stopping level, on one batch; a captured graph is dropped by the change and
+    captured again."""
     import hcorepp_amd as H
     n = 48
     ups, As, Bs, Cs, _ = _batch(H, ctx, n, nb, lo, hi, 11, min(nb, 2 * hi))
@@ -63,7 +64,12 @@ def test_panel_widths_agree(ctx, nb, lo, hi):
     rk_ref, d_ref = results[(16, 2)]
     for key, (rk, d) in results.items():
         assert np.array_equal(rk, rk_ref), key
-        assert max(rel_fro(a, r) for a, r in zip(d, d_ref)) < 1e-12, key
+        # the batched Jacobi stops once a sweep's cosines are <= 1e-5, leaving U orthonormal to
+        # about 1e-10; QRs that differ in rounding can stop a sweep apart, so the results agree
+        # to that level, not to the QR's roundoff
+        diff = max(rel_fro(a, r) for a, r in zip(d, d_ref))
+        print(f"panel {key}: max relative difference {diff:.2e}", flush=True)
+        assert diff < 1e-10, key
     b.close()
 
 
