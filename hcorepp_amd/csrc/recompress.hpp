@@ -318,7 +318,7 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, i
 // panels, with all three products on the matrix cores (v_mfma_f64_16x16x4f64).  Each wave owns
 // whole 16-column blocks of B, so the reduction over the rows happens inside the matrix core and
 // no barrier or cross-wave exchange is needed:
-//   W  = P^T B_blk    (16 x 16, K = rows: rows/4 MFMA steps in two interleaved chains)
+//   W  = P^T B_blk    (16 x 16, K = rows: rows/4 MFMA steps in four interleaved chains)
 //   W' = op(T) W      (4 MFMA steps: W's accumulator is already the B operand layout)
 //   B_blk -= P W'     (per 16 rows: 4 MFMA steps of K = 4)
 // MFMA lane layout (f64 16x16x4): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
@@ -344,33 +344,34 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
         const int k = 4 * s + k4;
         ta[s] = trans ? (k <= i16 ? Tm[k + 16 * i16] : 0.0) : (i16 <= k ? Tm[i16 + 16 * k] : 0.0);
     }
-    const int r32 = rows & ~31;
+    const int r64 = rows & ~63;
     const int prow = 4 * (i16 & 3) + (i16 >> 2);  // row of MFMA output row i16 within its group
     for (int blk = wave_id(); blk < nblk; blk += NWAVES) {
         const int col = c0 + 16 * blk + i16;
         const bool cok = col < c1;
         // lanes past c1 read column c0 (in range) and discard it
         double *bcol = B + (int64_t) (cok ? col : c0) * ldb;
-        hcamd_doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+        // W = P^T B_blk over 64-row chunks: the 16 loads of B and of P of a chunk are in flight
+        // together, and four accumulators (one per 16-row group) keep four MFMA chains apart
+        hcamd_doublex4 acc[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) acc[g] = hcamd_doublex4{0.0, 0.0, 0.0, 0.0};
         int r0 = 0;
-        for (; r0 < r32; r0 += 32) {
-            const int ra = r0 + 4 * k4, rb = ra + 16;
-            double b0[4], b1[4], a0[4], a1[4];
+        for (; r0 < r64; r0 += 64) {
+            double bv[4][4], av[4][4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                b0[s] = bcol[ra + s];
-                b1[s] = bcol[rb + s];
-            }
+            for (int g = 0; g < 4; ++g)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                a0[s] = P[ra + s + i16 * pld];
-                a1[s] = P[rb + s + i16 * pld];
-            }
+                for (int s = 0; s < 4; ++s) bv[g][s] = bcol[r0 + 16 * g + 4 * k4 + s];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s], cok ? b0[s] : 0.0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[s], cok ? b1[s] : 0.0, acc1, 0, 0, 0);
-            }
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) av[g][s] = P[r0 + 16 * g + 4 * k4 + s + i16 * pld];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][s], cok ? bv[g][s] : 0.0, acc[g], 0, 0, 0);
         }
         for (; r0 < rows; r0 += 16) {
             const int ra = r0 + 4 * k4;
@@ -379,18 +380,43 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
                 const int r = ra + s;
                 const double a = r < rows ? P[r + i16 * pld] : 0.0;
                 const double b = (r < rows && cok) ? bcol[r] : 0.0;
-                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[0], 0, 0, 0);
             }
         }
-        // acc[v] = W[4 v + k4][i16]: element s is the B operand of step s of W' = op(T) W
-        const hcamd_doublex4 acc = acc0 + acc1;
+        // w[v] = W[4 v + k4][i16]: element s is the B operand of step s of W' = op(T) W
+        const hcamd_doublex4 w = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         hcamd_doublex4 wp = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) wp = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[s], acc[s], wp, 0, 0, 0);
-        // wp[s] = W'[4 s + k4][i16], the B operand of step s of the update
-#pragma unroll 2
-        for (int g = 0; g < rows; g += 16) {
-            const int pa = g + prow, rq = g + 4 * k4;
+        for (int s = 0; s < 4; ++s) wp = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[s], w[s], wp, 0, 0, 0);
+        // wp[s] = W'[4 s + k4][i16], the B operand of step s of the update B_blk -= P W'; in
+        // 64-row chunks the 16 loads of B and of P are in flight together
+        int g0 = 0;
+        for (; g0 < r64; g0 += 64) {
+            double old[4][4], av[4][4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) old[g][v] = bcol[g0 + 16 * g + 4 * k4 + v];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) av[g][s] = P[g0 + 16 * g + prow + (4 * s + k4) * pld];
+            hcamd_doublex4 d[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) d[g] = hcamd_doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) d[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][s], wp[s], d[g], 0, 0, 0);
+            if (cok) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) bcol[g0 + 16 * g + 4 * k4 + v] = old[g][v] - d[g][v];
+            }
+        }
+        for (; g0 < rows; g0 += 16) {
+            const int pa = g0 + prow, rq = g0 + 4 * k4;
             double old[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) old[v] = (rq + v < rows) ? bcol[rq + v] : 0.0;
