@@ -12,66 +12,98 @@ namespace hcamd {
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
 constexpr int GT_LDS_ELEMS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
 
-template <typename T>
-__device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B,
-                             int64_t ldb, bool tb, T beta, T *C, int64_t ldc, int ti, int tj, T *lds,
-                             bool upper_b = false) {
+// The K loop is software-pipelined: the next chunk's elements are loaded into registers (four of
+// A and four of B per thread) right after the current chunk is in LDS, so their memory latency
+// overlaps the chunk's 16 rank-1 steps.  `fa(i, k)` and `fb(k, j)` return op(A)[i, k] and
+// op(B)[k, j] (zero outside the operands); A's chunk is read row-fastest when `a_rows_fast`, B's
+// K-fastest unless `b_cols_fast`, so that consecutive threads read consecutive addresses.
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA &fa, const FB &fb, bool a_rows_fast,
+                                                  bool b_cols_fast, T (&acc)[4][4], int ti, int tj, T *lds) {
     T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
     T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
     const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
     const int i0 = ti * GT_M, j0 = tj * GT_N;
-    T acc[4][4];
+    constexpr int PER = GT_M * GT_K / NT;  // elements of each operand chunk per thread (4)
+    static_assert(GT_M * GT_K == GT_N * GT_K && PER * NT == GT_M * GT_K, "tile shape");
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int p = 0; p < 4; ++p)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-    for (int k0 = 0; k0 < Kd; k0 += GT_K) {
-        // A tile: i in [i0, i0+64), k in [k0, k0+16)
-        for (int e = threadIdx.x; e < GT_M * GT_K; e += NT) {
-            int i, k;
-            if (!ta) {
-                i = e % GT_M;
-                k = e / GT_M;
-            } else {
-                k = e % GT_K;
-                i = e / GT_K;
-            }
-            const int gi = i0 + i, gk = k0 + k;
-            T v = T(0);
-            if (gi < M && gk < Kd) v = ta ? A[gk + (int64_t) gi * lda] : A[gi + (int64_t) gk * lda];
-            As[k * (GT_M + 1) + i] = v;
+        for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
+    auto a_pos = [&](int e, int &i, int &k) {
+        if (a_rows_fast) {
+            i = e % GT_M;
+            k = e / GT_M;
+        } else {
+            k = e % GT_K;
+            i = e / GT_K;
         }
-        for (int e = threadIdx.x; e < GT_N * GT_K; e += NT) {
-            int j, k;
-            if (!tb) {
-                k = e % GT_K;
-                j = e / GT_K;
-            } else {
-                j = e % GT_N;
-                k = e / GT_N;
-            }
-            const int gj = j0 + j, gk = k0 + k;
-            T v = T(0);
-            if (gj < N && gk < Kd && !(upper_b && gk < gj))
-                v = tb ? B[gj + (int64_t) gk * ldb] : B[gk + (int64_t) gj * ldb];
-            Bs[k * (GT_N + 1) + j] = v;
+    };
+    auto b_pos = [&](int e, int &j, int &k) {
+        if (b_cols_fast) {
+            j = e % GT_N;
+            k = e / GT_N;
+        } else {
+            k = e % GT_K;
+            j = e / GT_K;
+        }
+    };
+    T ra[PER], rb[PER];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            int i, j, k, k2;
+            a_pos(e, i, k);
+            b_pos(e, j, k2);
+            const int gi = i0 + i, gk = k0 + k, gj = j0 + j, gk2 = k0 + k2;
+            ra[q] = (gi < M && gk < Kd) ? fa(gi, gk) : T(0);
+            rb[q] = (gj < N && gk2 < Kd) ? fb(gk2, gj) : T(0);
+        }
+    };
+    if (Kd > 0) load(0);
+    for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            int i, j, k, k2;
+            a_pos(e, i, k);
+            b_pos(e, j, k2);
+            As[k * (GT_M + 1) + i] = ra[q];
+            Bs[k2 * (GT_N + 1) + j] = rb[q];
         }
         __syncthreads();
+        if (k0 + GT_K < Kd) load(k0 + GT_K);
 #pragma unroll
         for (int k = 0; k < GT_K; ++k) {
-            T a[4], b[4];
+            T av[4], bv[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                a[q] = As[k * (GT_M + 1) + ty + 16 * q];
-                b[q] = Bs[k * (GT_N + 1) + tx + 16 * q];
+                av[q] = As[k * (GT_M + 1) + ty + 16 * q];
+                bv[q] = Bs[k * (GT_N + 1) + tx + 16 * q];
             }
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+                for (int q = 0; q < 4; ++q) acc[p][q] += av[p] * bv[q];
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t lda, bool ta, const T *B,
+                             int64_t ldb, bool tb, T beta, T *C, int64_t ldc, int ti, int tj, T *lds,
+                             bool upper_b = false) {
+    auto fa = [=](int i, int k) -> T { return ta ? A[k + (int64_t) i * lda] : A[i + (int64_t) k * lda]; };
+    auto fb = [=](int k, int j) -> T {
+        if (upper_b && k < j) return T(0);
+        return tb ? B[j + (int64_t) k * ldb] : B[k + (int64_t) j * ldb];
+    };
+    T acc[4][4];
+    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, !ta, tb, acc, ti, tj, lds);
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int i0 = ti * GT_M, j0 = tj * GT_N;
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const int gi = i0 + ty + 16 * p;
@@ -92,42 +124,10 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
 template <typename T, typename FA, typename FB>
 __device__ void wg_gemm_tile_fn(int M, int N, int Kd, const FA &fa, const FB &fb, T *C, int64_t ldc, int ti, int tj,
                                 T *lds) {
-    T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
-    T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
+    T acc[4][4];
+    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, true, false, acc, ti, tj, lds);
     const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
     const int i0 = ti * GT_M, j0 = tj * GT_N;
-    T acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-    for (int k0 = 0; k0 < Kd; k0 += GT_K) {
-        for (int e = threadIdx.x; e < GT_M * GT_K; e += NT) {
-            const int i = e % GT_M, k = e / GT_M;
-            const int gi = i0 + i, gk = k0 + k;
-            As[k * (GT_M + 1) + i] = (gi < M && gk < Kd) ? fa(gi, gk) : T(0);
-        }
-        for (int e = threadIdx.x; e < GT_N * GT_K; e += NT) {
-            const int k = e % GT_K, j = e / GT_K;
-            const int gj = j0 + j, gk = k0 + k;
-            Bs[k * (GT_N + 1) + j] = (gj < N && gk < Kd) ? fb(gk, gj) : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < GT_K; ++k) {
-            T a[4], b[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                a[q] = As[k * (GT_M + 1) + ty + 16 * q];
-                b[q] = Bs[k * (GT_N + 1) + tx + 16 * q];
-            }
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
-        }
-        __syncthreads();
-    }
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const int gi = i0 + ty + 16 * p;
