@@ -239,6 +239,9 @@ __device__ __forceinline__ int wave_multi_sum_index(int l) {
     return l >> (6 - log2_const<NV>());
 }
 
+// Accumulator of one FP64 matrix-core operation (v_mfma_f64_16x16x4f64): four doubles per lane.
+typedef double hcamd_doublex4 __attribute__((ext_vector_type(4)));
+
 // dst[e] = src[e] for e in [0, n) by the whole workgroup, four loads in flight per thread.
 template <typename T>
 __device__ __forceinline__ void wg_copy(T *dst, const T *src, int64_t n) {

@@ -14,22 +14,23 @@ constexpr int GT_LDS_ELEMS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
 
 // The K loop is software-pipelined: the next chunk's elements are loaded into registers (four of
 // A and four of B per thread) right after the current chunk is in LDS, so their memory latency
-// overlaps the chunk's 16 rank-1 steps.  `fa(i, k)` and `fb(k, j)` return op(A)[i, k] and
-// op(B)[k, j] (zero outside the operands); A's chunk is read row-fastest when `a_rows_fast`, B's
-// K-fastest unless `b_cols_fast`, so that consecutive threads read consecutive addresses.
-template <typename T, typename FA, typename FB>
+// overlaps the chunk's arithmetic.  `fa(i, k)` and `fb(k, j)` return op(A)[i, k] and op(B)[k, j]
+// (zero outside the operands); A's chunk is read row-fastest when `a_rows_fast`, B's K-fastest
+// unless `b_cols_fast`, so that consecutive threads read consecutive addresses.  `epi(i, j, v)`
+// receives each element of the tile inside M x N.
+// FP64 runs on the matrix cores (v_mfma_f64_16x16x4f64): wave w computes the 32 x 32 quarter at
+// rows 32 (w / 2), columns 32 (w % 2) of the tile as 2 x 2 blocks of 16 x 16; per 4-deep K step
+// a lane reads two values of A and two of B from LDS for four MFMAs (the VALU form, a 4 x 4
+// block per thread, reads eight for sixteen FMAs).  MFMA lane layout: A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[4 v + l / 16][l % 16] in element v of lane l.
+template <typename T, typename FA, typename FB, typename EPI>
 __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA &fa, const FB &fb, bool a_rows_fast,
-                                                  bool b_cols_fast, T (&acc)[4][4], int ti, int tj, T *lds) {
+                                                  bool b_cols_fast, const EPI &epi, int ti, int tj, T *lds) {
     T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
     T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
     const int i0 = ti * GT_M, j0 = tj * GT_N;
     constexpr int PER = GT_M * GT_K / NT;  // elements of each operand chunk per thread (4)
     static_assert(GT_M * GT_K == GT_N * GT_K && PER * NT == GT_M * GT_K, "tile shape");
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
     auto a_pos = [&](int e, int &i, int &k) {
         if (a_rows_fast) {
             i = e % GT_M;
@@ -61,8 +62,7 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
             rb[q] = (gj < N && gk2 < Kd) ? fb(gk2, gj) : T(0);
         }
     };
-    if (Kd > 0) load(0);
-    for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+    auto stage = [&]() {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
@@ -72,22 +72,79 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
             As[k * (GT_M + 1) + i] = ra[q];
             Bs[k2 * (GT_N + 1) + j] = rb[q];
         }
-        __syncthreads();
-        if (k0 + GT_K < Kd) load(k0 + GT_K);
+    };
+    if constexpr (sizeof(T) == 8) {
+        const int l = lane_id(), i16 = l & 15, k4 = l >> 4, w = wave_id();
+        const int qi = 32 * (w >> 1), qj = 32 * (w & 1);
+        hcamd_doublex4 acc[2][2];
 #pragma unroll
-        for (int k = 0; k < GT_K; ++k) {
-            T av[4], bv[4];
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = hcamd_doublex4{0.0, 0.0, 0.0, 0.0};
+        if (Kd > 0) load(0);
+        for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+            stage();
+            __syncthreads();
+            if (k0 + GT_K < Kd) load(k0 + GT_K);
+#pragma unroll
+            for (int s = 0; s < GT_K / 4; ++s) {
+                const int kr = 4 * s + k4;
+                double av[2], bv[2];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi) av[bi] = As[kr * (GT_M + 1) + qi + 16 * bi + i16];
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) bv[bj] = Bs[kr * (GT_N + 1) + qj + 16 * bj + i16];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[bi], bv[bj], acc[bi][bj], 0, 0, 0);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int gi = i0 + qi + 16 * bi + 4 * v + k4, gj = j0 + qj + 16 * bj + i16;
+                    if (gi < M && gj < N) epi(gi, gj, acc[bi][bj][v]);
+                }
+    } else {
+        const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+        T acc[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
+        if (Kd > 0) load(0);
+        for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+            stage();
+            __syncthreads();
+            if (k0 + GT_K < Kd) load(k0 + GT_K);
+#pragma unroll
+            for (int k = 0; k < GT_K; ++k) {
+                T av[4], bv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    av[q] = As[k * (GT_M + 1) + ty + 16 * q];
+                    bv[q] = Bs[k * (GT_N + 1) + tx + 16 * q];
+                }
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[p][q] += av[p] * bv[q];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                av[q] = As[k * (GT_M + 1) + ty + 16 * q];
-                bv[q] = Bs[k * (GT_N + 1) + tx + 16 * q];
+                const int gi = i0 + ty + 16 * p, gj = j0 + tx + 16 * q;
+                if (gi < M && gj < N) epi(gi, gj, acc[p][q]);
             }
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[p][q] += av[p] * bv[q];
-        }
-        __syncthreads();
     }
 }
 
@@ -100,22 +157,11 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
         if (upper_b && k < j) return T(0);
         return tb ? B[j + (int64_t) k * ldb] : B[k + (int64_t) j * ldb];
     };
-    T acc[4][4];
-    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, !ta, tb, acc, ti, tj, lds);
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    const int i0 = ti * GT_M, j0 = tj * GT_N;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const int gi = i0 + ty + 16 * p;
-        if (gi >= M) continue;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int gj = j0 + tx + 16 * q;
-            if (gj >= N) continue;
-            T *c = C + gi + (int64_t) gj * ldc;
-            *c = (beta == T(0)) ? alpha * acc[p][q] : alpha * acc[p][q] + beta * *c;
-        }
-    }
+    auto epi = [=](int gi, int gj, T v) {
+        T *c = C + gi + (int64_t) gj * ldc;
+        *c = (beta == T(0)) ? alpha * v : alpha * v + beta * *c;
+    };
+    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, !ta, tb, epi, ti, tj, lds);
 }
 
 // One 64 x 64 tile of C (M x N, ld ldc) = sum_k fa(i, k) fb(k, j) for operands given as element
@@ -124,20 +170,8 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
 template <typename T, typename FA, typename FB>
 __device__ void wg_gemm_tile_fn(int M, int N, int Kd, const FA &fa, const FB &fb, T *C, int64_t ldc, int ti, int tj,
                                 T *lds) {
-    T acc[4][4];
-    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, true, false, acc, ti, tj, lds);
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    const int i0 = ti * GT_M, j0 = tj * GT_N;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const int gi = i0 + ty + 16 * p;
-        if (gi >= M) continue;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int gj = j0 + tx + 16 * q;
-            if (gj < N) C[gi + (int64_t) gj * ldc] = acc[p][q];
-        }
-    }
+    auto epi = [=](int gi, int gj, T v) { C[gi + (int64_t) gj * ldc] = v; };
+    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, true, false, epi, ti, tj, lds);
 }
 
 // Whole-matrix GEMM by one workgroup (loops over all 64x64 tiles).

@@ -331,8 +331,6 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, i
 // consecutive elements of its column per group.  P: panel in LDS (rows x 16, ld pld, unit
 // lower); Tm: 16 x 16 upper (LDS, zero below the diagonal).
 // ---------------------------------------------------------------------------
-typedef double hcamd_doublex4 __attribute__((ext_vector_type(4)));
-
 __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const double *Tm, int rows, double *B,
                                                int64_t ldb, int c0, int c1, bool trans) {
     const int l = lane_id(), i16 = l & 15, k4 = l >> 4;
@@ -647,33 +645,41 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
         for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
     }
     __syncthreads();
-    // Householder steps with one pass over the rows and two barriers per column: column jj's
-    // partial products with itself and with the later panel columns are reduced together, which
-    // gives its norm and the dot products of the (still unscaled) reflector with those columns.
-    // Every lane then forms the reflector and the update coefficients itself: lane k holds the
-    // sums of value k, and readlane spreads them.  Row jj is copied to `xrow` before the barrier
-    // because its owner updates it in place.
-    T *xrow = red + NWAVES * PB;
+    // Householder steps with one pass over the rows and one barrier per column.  The pass of step
+    // jj applies reflector jj to the rows below it and, on the updated values, accumulates column
+    // jj+1's products with itself and with the later panel columns: reduced, they give the next
+    // column's norm and the dot products of its (still unscaled) reflector with those columns.
+    // Every lane then forms the reflector and the update coefficients itself (lane k holds the
+    // sums of value k; readlane spreads them).  The reduction slots and the copy of the next
+    // pivot row (`xrow`, written by that row's owner) alternate between two buffers by step
+    // parity, so one barrier per step separates their writers and readers.
     const int kl = l & (PB - 1);
-    for (int jj = 0; jj < jb; ++jj) {
+    auto reduce_next = [&](T (&acc)[PB], int buf) {
+        const T s = wave_multi_sum_fast<T, PB>(acc);
+        const int idx = wave_multi_sum_index<PB>(l);
+        if (l == multi_sum_src_lane<PB>(idx)) red[buf * (NWAVES * PB) + wave_id() * PB + idx] = s;
+    };
+    T *xrow = red + 2 * NWAVES * PB;  // [2][PB]
+    {
         T acc[PB];
 #pragma unroll
         for (int k = 0; k < PB; ++k) acc[k] = T(0);
-        for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
-            const T x = P[r + jj * pld];
+        for (int r = 1 + threadIdx.x; r < rows; r += NT) {
+            const T x = P[r];
 #pragma unroll
             for (int k = 0; k < PB; ++k)
-                if (k >= jj && k < jb) acc[k] += x * P[r + k * pld];
+                if (k < jb) acc[k] += x * P[r + k * pld];
         }
-        const T s = wave_multi_sum_fast<T, PB>(acc);
-        const int idx = wave_multi_sum_index<PB>(l);
-        if (l == multi_sum_src_lane<PB>(idx)) red[wave_id() * PB + idx] = s;
-        if (threadIdx.x < PB) xrow[threadIdx.x] = P[jj + threadIdx.x * pld];
+        reduce_next(acc, 0);
+        if (threadIdx.x < PB) xrow[threadIdx.x] = P[threadIdx.x * pld];
         __syncthreads();
+    }
+    for (int jj = 0; jj < jb; ++jj) {
+        const int buf = jj & 1;
         T sk = T(0);
 #pragma unroll
-        for (int wv = 0; wv < NWAVES; ++wv) sk += red[wv * PB + kl];
-        const T xk = xrow[kl];
+        for (int wv = 0; wv < NWAVES; ++wv) sk += red[buf * (NWAVES * PB) + wv * PB + kl];
+        const T xk = xrow[buf * PB + kl];
         const T xnorm2 = read_lane(sk, jj), alpha = read_lane(xk, jj);
         T tau_j, beta, scal;
         if (xnorm2 == T(0)) {
@@ -697,15 +703,41 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
             for (int k = 0; k < PB; ++k)
                 if (k > jj && k < jb) P[jj + k * pld] -= g[k];
         }
-        if (tau_j != T(0)) {
-            for (int r = jj + 1 + threadIdx.x; r < rows; r += NT) {
+        const int nj = jj + 1;
+        const bool more = nj < jb;
+        T acc[PB];
+#pragma unroll
+        for (int k = 0; k < PB; ++k) acc[k] = T(0);
+        for (int r = nj + threadIdx.x; r < rows; r += NT) {
+            T x[PB];
+#pragma unroll
+            for (int k = 0; k < PB; ++k) x[k] = P[r + k * pld];
+            if (tau_j != T(0)) {
                 const T v = scal * P[r + jj * pld];
                 P[r + jj * pld] = v;
 #pragma unroll
                 for (int k = 0; k < PB; ++k)
-                    if (k > jj && k < jb) P[r + k * pld] -= g[k] * v;
+                    if (k > jj && k < jb) {
+                        x[k] -= g[k] * v;
+                        P[r + k * pld] = x[k];
+                    }
+            }
+            if (more) {
+                if (r == nj) {
+#pragma unroll
+                    for (int k = 0; k < PB; ++k) xrow[(buf ^ 1) * PB + k] = x[k];
+                } else {
+                    T xn = T(0);  // x[nj] without a register-indexed read
+#pragma unroll
+                    for (int k = 0; k < PB; ++k)
+                        if (k == nj) xn = x[k];
+#pragma unroll
+                    for (int k = 0; k < PB; ++k)
+                        if (k >= nj && k < jb) acc[k] += xn * x[k];
+                }
             }
         }
+        if (more) reduce_next(acc, buf ^ 1);
         __syncthreads();
     }
     HCAMD_SUB_END(t_pan, hdr, 0);
