@@ -452,18 +452,55 @@ __device__ void apply_wy_dispatch(const T *P, int pld, const T *Tm, int rows, T 
 
 // Build T (PB x PB, upper, column-major in LDS) for the panel P with taus tau[0..jb):
 // dlarft forward/columnwise: T[i][i] = tau_i, T[0:i, i] = -tau_i T[0:i,0:i] (V[:,0:i]^T v_i).
+// For FP64 panels of 16 columns the Gram matrix P^T P is formed on the matrix cores: wave w sums
+// the 16-row groups w, w + 4, ... (A = P^T and B = P take the same lane values), and waves 1..3
+// pass their partial sums to wave 0 through the 3 x 256 elements from Tm on (the T factor, Gram
+// and reduction areas, which follow one another in the panel workspace and are free here).
 template <typename T, int PB>
 __device__ void build_t(const T *P, int pld, int rows, const T *tau, int jb, T *Tm, T *gram) {
     const int l = lane_id();
-    for (int pr = wave_id(); pr < PB * PB; pr += NWAVES) {
-        const int a = pr % PB, b = pr / PB;
-        if (a >= b || b >= jb) continue;
-        T s = T(0);
-        for (int r = l; r < rows; r += WAVE) s += P[r + a * pld] * P[r + b * pld];
-        s = wave_sum(s);
-        if (l == 0) gram[a + b * PB] = s;
+    if constexpr (sizeof(T) == 8 && PB == 16) {
+        const int i16 = l & 15, k4 = l >> 4, w = wave_id();
+        hcamd_doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int r0 = 16 * w; r0 < rows; r0 += 16 * NWAVES) {
+            double a[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int r = r0 + 4 * k4 + s;
+                a[s] = r < rows ? P[r + i16 * pld] : 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], a[s], acc, 0, 0, 0);
+        }
+        // element v of lane l is G[4 v + k4][i16], kept at (4 v + k4) + 16 i16
+        T *part = Tm;
+        if (w > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) part[(w - 1) * 256 + (4 * v + k4) + 16 * i16] = acc[v];
+        }
+        __syncthreads();
+        if (w == 0) {
+            double g[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int e = (4 * v + k4) + 16 * i16;
+                g[v] = acc[v] + part[e] + part[256 + e] + part[512 + e];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gram[(4 * v + k4) + 16 * i16] = g[v];
+        }
+        __syncthreads();
+    } else {
+        for (int pr = wave_id(); pr < PB * PB; pr += NWAVES) {
+            const int a = pr % PB, b = pr / PB;
+            if (a >= b || b >= jb) continue;
+            T s = T(0);
+            for (int r = l; r < rows; r += WAVE) s += P[r + a * pld] * P[r + b * pld];
+            s = wave_sum(s);
+            if (l == 0) gram[a + b * PB] = s;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // Lane a < PB builds row a of T: T[a][i] = -tau_i sum_{b=a}^{i-1} T[a][b] gram[b][i] only
     // reads row a, so the rows are independent.
     if (threadIdx.x < PB) {
