@@ -115,27 +115,28 @@ __host__ __device__ __forceinline__ constexpr int jslot_col(int rd, int slot) {
 template <typename T, int RPL, int RPLV, int TW, bool FULL>
 __device__ __forceinline__ int jacobi_round_fixed(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], T tol, T floor2) {
     constexpr int NP = TW / 2;
-    constexpr int NV = (3 * NP <= 4) ? 4 : (3 * NP <= 8) ? 8 : (3 * NP <= 16) ? 16 : 32;
     const int l = lane_id();
-    T part[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) part[k] = T(0);
+    // the squared norms (2 NP values) and the cross products (NP values) are reduced separately:
+    // both counts are powers of two, so no padded slot takes part in the exchanges
+    T nrm[2 * NP], crs[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int q = jslot_q<T, TW, FULL>(k);
+        nrm[k] = nrm[NP + k] = crs[k] = T(0);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const T a = y[t][k], b = y[t][q];
-            part[3 * k] += a * a;
-            part[3 * k + 1] += b * b;
-            part[3 * k + 2] += a * b;
+            nrm[k] += a * a;
+            nrm[NP + k] += b * b;
+            crs[k] += a * b;
         }
     }
-    const T tot = wave_multi_sum_fast<T, NV>(part);
+    const T tn = wave_multi_sum_fast<T, 2 * NP>(nrm);
+    const T tc = wave_multi_sum_fast<T, NP>(crs);
     const int kk = l < NP ? l : 0;
-    const T app = __shfl(tot, multi_sum_src_lane<NV>(3 * kk), WAVE);
-    const T aqq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 1), WAVE);
-    const T apq = __shfl(tot, multi_sum_src_lane<NV>(3 * kk + 2), WAVE);
+    const T app = __shfl(tn, multi_sum_src_lane<2 * NP>(kk), WAVE);
+    const T aqq = __shfl(tn, multi_sum_src_lane<2 * NP>(NP + kk), WAVE);
+    const T apq = __shfl(tc, multi_sum_src_lane<NP>(kk), WAVE);
     T c, s;
     int lg;
     jacobi_rotation<T, (RPLV > 0)>(app, aqq, apq, tol, floor2, c, s, lg);
@@ -187,11 +188,10 @@ __device__ __forceinline__ void jacobi_rotate_slots(T (&y)[ROWS][TW]) {
 // per lane) are loaded together, so the tile costs one memory round trip; columns >= ncols
 // are zero and never stored.
 template <typename T, int RPL, int RPLV, int TW>
-__device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ, bool full,
-                           T tol, T floor2) {
+__device__ __forceinline__ void jtile_load(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], const T *J, int rows,
+                                           int ncols, int64_t ldj, const T *V, int64_t ldv, int cI, int cJ) {
     constexpr int B = TW / 2;
     const int l = lane_id();
-    T y[RPL][TW], vt[vrows<RPLV>()][TW];
 #pragma unroll
     for (int c = 0; c < TW; ++c) {
         const int gc = (c < B) ? cI + c : cJ + (c - B);
@@ -206,6 +206,33 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
             vt[t][c] = (r < ncols && gc < ncols) ? V[r + (int64_t) gc * ldv] : T(0);
         }
     }
+}
+
+template <typename T, int RPL, int RPLV, int TW>
+__device__ __forceinline__ void jtile_store(const T (&y)[RPL][TW], const T (&vt)[vrows<RPLV>()][TW], T *J, int rows,
+                                            int ncols, int64_t ldj, T *V, int64_t ldv, int cI, int cJ) {
+    constexpr int B = TW / 2;
+    const int l = lane_id();
+#pragma unroll
+    for (int c = 0; c < TW; ++c) {
+        const int gc = (c < B) ? cI + c : cJ + (c - B);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int r = l + WAVE * t;
+            if (r < rows && gc < ncols) J[r + (int64_t) gc * ldj] = y[t][c];
+        }
+#pragma unroll
+        for (int t = 0; t < RPLV; ++t) {
+            const int r = l + WAVE * t;
+            if (r < ncols && gc < ncols) V[r + (int64_t) gc * ldv] = vt[t][c];
+        }
+    }
+}
+
+// The rounds of a full (all pairs of the tile's columns) or cross (pairs between its two
+// blocks) tile; returns the JROT flags.
+template <typename T, int RPL, int RPLV, int TW>
+__device__ __forceinline__ int jtile_rounds(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], bool full, T tol, T floor2) {
     int any = 0;
     if (full) {
 #pragma unroll 1
@@ -222,23 +249,28 @@ __device__ int jacobi_tile(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t
             if constexpr (RPLV > 0) jacobi_rotate_slots<T, RPLV, TW, false>(vt);
         }
     }
-    if (any & JROT_APPLIED) {
-#pragma unroll
-        for (int c = 0; c < TW; ++c) {
-            const int gc = (c < B) ? cI + c : cJ + (c - B);
-#pragma unroll
-            for (int t = 0; t < RPL; ++t) {
-                const int r = l + WAVE * t;
-                if (r < rows && gc < ncols) J[r + (int64_t) gc * ldj] = y[t][c];
-            }
-#pragma unroll
-            for (int t = 0; t < RPLV; ++t) {
-                const int r = l + WAVE * t;
-                if (r < ncols && gc < ncols) V[r + (int64_t) gc * ldv] = vt[t][c];
-            }
-        }
-    }
     return any;
+}
+
+// Block pair bp of round-robin step `step` over nbp blocks of B columns: its first columns (cI,
+// cJ) and whether it exists (the padding block of an odd block count pairs with nothing).
+__device__ __forceinline__ bool jacobi_pair_cols(int bp, int step, int nbp, int B, int ncols, int &cI, int &cJ) {
+    int bI, bJ;
+    if (bp == 0) {
+        bI = nbp - 1;
+        bJ = step;
+    } else {
+        bI = (step + bp) % (nbp - 1);
+        bJ = (step - bp + nbp - 1) % (nbp - 1);
+    }
+    if (bI > bJ) {
+        const int t = bI;
+        bI = bJ;
+        bJ = t;
+    }
+    cI = bI * B;
+    cJ = bJ * B;
+    return cI < ncols;
 }
 
 // LDS elements wg_block_jacobi needs: a reduction slot per wave and a flag.
@@ -252,7 +284,7 @@ __host__ __device__ constexpr int block_jacobi_lds_elems() {
 // `gbuf` is LDS of block_jacobi_lds_elems<T, TW>() elements (all the LDS this routine touches,
 // so it may be called out of line).  Returns the sweeps used (max_sweeps when it did not
 // converge).
-template <typename T, int RPL, int RPLV, int TW>
+template <typename T, int RPL, int RPLV, int TW, bool PREFETCH = false>
 __device__ __forceinline__ int wg_block_jacobi(T *J, int rows, int ncols, int64_t ldj, T *V, int64_t ldv,
                                                int max_sweeps, T *gbuf) {
     constexpr int B = TW / 2;
@@ -276,23 +308,42 @@ __device__ __forceinline__ int wg_block_jacobi(T *J, int rows, int ncols, int64_
         __syncthreads();
         int any = 0;
         for (int step = 0; step < nbp - 1; ++step) {
-            for (int bp = wave_id(); bp < nbp / 2; bp += NWAVES) {
-                int bI, bJ;
-                if (bp == 0) {
-                    bI = nbp - 1;
-                    bJ = step;
-                } else {
-                    bI = (step + bp) % (nbp - 1);
-                    bJ = (step - bp + nbp - 1) % (nbp - 1);
+            // the wave's tiles of one step are disjoint: with PREFETCH the next one is loaded
+            // while the current one's rounds run (for J in global memory, where a load is a
+            // memory round trip)
+            int bp = wave_id(), cI = 0, cJ = 0;
+            while (bp < nbp / 2 && !jacobi_pair_cols(bp, step, nbp, B, ncols, cI, cJ)) bp += NWAVES;
+            if constexpr (PREFETCH) {
+                T yn[RPL][TW], vn[vrows<RPLV>()][TW];
+                if (bp < nbp / 2) jtile_load<T, RPL, RPLV, TW>(yn, vn, J, rows, ncols, ldj, V, ldv, cI, cJ);
+                while (bp < nbp / 2) {
+                    T y[RPL][TW], vt[vrows<RPLV>()][TW];
+#pragma unroll
+                    for (int t = 0; t < RPL; ++t)
+#pragma unroll
+                        for (int c = 0; c < TW; ++c) y[t][c] = yn[t][c];
+#pragma unroll
+                    for (int t = 0; t < vrows<RPLV>(); ++t)
+#pragma unroll
+                        for (int c = 0; c < TW; ++c) vt[t][c] = vn[t][c];
+                    const int c0 = cI, c1 = cJ;
+                    bp += NWAVES;
+                    while (bp < nbp / 2 && !jacobi_pair_cols(bp, step, nbp, B, ncols, cI, cJ)) bp += NWAVES;
+                    if (bp < nbp / 2) jtile_load<T, RPL, RPLV, TW>(yn, vn, J, rows, ncols, ldj, V, ldv, cI, cJ);
+                    const int a = jtile_rounds<T, RPL, RPLV, TW>(y, vt, step == 0, tol, floor2);
+                    if (a & JROT_APPLIED) jtile_store<T, RPL, RPLV, TW>(y, vt, J, rows, ncols, ldj, V, ldv, c0, c1);
+                    any |= a;
                 }
-                if (bI > bJ) {
-                    const int t = bI;
-                    bI = bJ;
-                    bJ = t;
+            } else {
+                while (bp < nbp / 2) {
+                    T y[RPL][TW], vt[vrows<RPLV>()][TW];
+                    jtile_load<T, RPL, RPLV, TW>(y, vt, J, rows, ncols, ldj, V, ldv, cI, cJ);
+                    const int a = jtile_rounds<T, RPL, RPLV, TW>(y, vt, step == 0, tol, floor2);
+                    if (a & JROT_APPLIED) jtile_store<T, RPL, RPLV, TW>(y, vt, J, rows, ncols, ldj, V, ldv, cI, cJ);
+                    any |= a;
+                    bp += NWAVES;
+                    while (bp < nbp / 2 && !jacobi_pair_cols(bp, step, nbp, B, ncols, cI, cJ)) bp += NWAVES;
                 }
-                if (bI * B >= ncols) continue;
-                any |= jacobi_tile<T, RPL, RPLV, TW>(J, rows, ncols, ldj, V, ldv, bI * B, bJ * B, step == 0, tol,
-                                                     floor2);
             }
             __syncthreads();
         }
