@@ -88,16 +88,34 @@ class TLRCholesky:
                 progress(k)
         self.ctx.sync()
 
-    # Steps with at most this many trailing updates run them one by one in the plain form (whose
-    # QR and Jacobi phases spread over the whole GPU): a persistent batch of a few large updates
-    # lasts as long as its largest update on one workgroup (~100-150 ms at nb = 512, K = 340), the
-    # plain form ~15 ms per update.
-    PLAIN_STEP_MAX_UPDATES = 7
+    # Steps with at most this many trailing updates run them in the plain form (whose QR and
+    # Jacobi phases spread over the whole GPU), PLAIN_STREAMS at a time on contexts of their own: a
+    # persistent batch lasts as long as its largest update on one workgroup (~80 ms at nb = 512
+    # when the core keeps all K = 340 columns), the plain form takes a fraction of that per update.
+    PLAIN_STEP_MAX_UPDATES = 16
+    PLAIN_STREAMS = 4
+
+    def _plain_updates(self, k, pairs, params):
+        """The trailing updates of step k in the plain form, spread over PLAIN_STREAMS contexts
+        (distinct C tiles, so independent); the main stream's work is finished first."""
+        import concurrent.futures as cf
+        from . import RunContext
+        self.ctx.sync()
+        if not hasattr(self, "_aux"):
+            self._aux = [RunContext(self.ctx.device) for _ in range(self.PLAIN_STREAMS)]
+
+        def share(t):
+            for i, j in pairs[t::self.PLAIN_STREAMS]:
+                HCore.gemm(-1.0, self.L[(i, k)], self.LT[(j, k)], 1.0, self.L[(i, j)], self._aux[t], params)
+
+        n = min(self.PLAIN_STREAMS, len(pairs))
+        with cf.ThreadPoolExecutor(max_workers=n) as ex:
+            list(ex.map(share, range(n)))
 
     def factor_batched(self, params: CompressionParameters, precision=PREC_FP64, tol=0.0):
         """Per step one launch per tile operation kind; the trailing updates as one persistent
         recompression batch (optionally with the pivoted-R tolerance rule); the last steps, with
-        a handful of updates, in the plain form (FP64 exact core)."""
+        a handful of updates, in the plain form (FP64, exact core) on several streams."""
         for k in range(self.T):
             rows = list(range(k + 1, self.T))
             self._potrf(k)
@@ -107,9 +125,8 @@ class TLRCholesky:
             self._syrk(k, rows)
             self._transpose(k, rows)
             pairs = [(i, j) for j in rows for i in range(j + 1, self.T)]
-            if 0 < len(pairs) <= self.PLAIN_STEP_MAX_UPDATES and precision == PREC_FP64 and tol == 0.0:
-                for i, j in pairs:
-                    HCore.gemm(-1.0, self.L[(i, k)], self.LT[(j, k)], 1.0, self.L[(i, j)], self.ctx, params)
+            if 0 < len(pairs) <= self.PLAIN_STEP_MAX_UPDATES:
+                self._plain_updates(k, pairs, params)
             elif pairs:
                 b = Batch(self.ctx, [self.L[(i, k)] for i, j in pairs], [self.LT[(j, k)] for i, j in pairs],
                           [self.L[(i, j)] for i, j in pairs], -1.0, 1.0, params, precision)
