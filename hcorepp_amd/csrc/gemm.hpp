@@ -23,9 +23,14 @@ constexpr int GT_LDS_ELEMS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
 // a lane reads two values of A and two of B from LDS for four MFMAs (the VALU form, a 4 x 4
 // block per thread, reads eight for sixteen FMAs).  MFMA lane layout: A[i][k] in lane i + 16 k,
 // B[k][j] in lane j + 16 k, D[4 v + l / 16][l % 16] in element v of lane l.
+// Only K chunks overlapping [k_lo, k_hi) are visited (operands known to vanish outside, as for
+// triangular factors); k_lo is rounded down to a chunk boundary.
 template <typename T, typename FA, typename FB, typename EPI>
 __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA &fa, const FB &fb, bool a_rows_fast,
-                                                  bool b_cols_fast, const EPI &epi, int ti, int tj, T *lds) {
+                                                  bool b_cols_fast, const EPI &epi, int ti, int tj, T *lds,
+                                                  int k_lo = 0, int k_hi = -1) {
+    k_lo = (k_lo / GT_K) * GT_K;
+    if (k_hi < 0 || k_hi > Kd) k_hi = Kd;
     T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
     T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
     const int i0 = ti * GT_M, j0 = tj * GT_N;
@@ -81,11 +86,11 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = hcamd_doublex4{0.0, 0.0, 0.0, 0.0};
-        if (Kd > 0) load(0);
-        for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+        if (k_lo < k_hi) load(k_lo);
+        for (int k0 = k_lo; k0 < k_hi; k0 += GT_K) {
             stage();
             __syncthreads();
-            if (k0 + GT_K < Kd) load(k0 + GT_K);
+            if (k0 + GT_K < k_hi) load(k0 + GT_K);
 #pragma unroll
             for (int s = 0; s < GT_K / 4; ++s) {
                 const int kr = 4 * s + k4;
@@ -118,11 +123,11 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
         for (int p = 0; p < 4; ++p)
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
-        if (Kd > 0) load(0);
-        for (int k0 = 0; k0 < Kd; k0 += GT_K) {
+        if (k_lo < k_hi) load(k_lo);
+        for (int k0 = k_lo; k0 < k_hi; k0 += GT_K) {
             stage();
             __syncthreads();
-            if (k0 + GT_K < Kd) load(k0 + GT_K);
+            if (k0 + GT_K < k_hi) load(k0 + GT_K);
 #pragma unroll
             for (int k = 0; k < GT_K; ++k) {
                 T av[4], bv[4];
@@ -161,7 +166,8 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
         T *c = C + gi + (int64_t) gj * ldc;
         *c = (beta == T(0)) ? alpha * v : alpha * v + beta * *c;
     };
-    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, !ta, tb, epi, ti, tj, lds);
+    // with upper_b, op(B)(k, j) vanishes for k < j: the tile's columns start at tj * GT_N
+    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, !ta, tb, epi, ti, tj, lds, upper_b ? tj * GT_N : 0);
 }
 
 // One 64 x 64 tile of C (M x N, ld ldc) = sum_k fa(i, k) fb(k, j) for operands given as element
@@ -169,9 +175,9 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
 // chunks are staged in LDS as in wg_gemm_tile.  A's chunk is read row-fastest, B's K-fastest.
 template <typename T, typename FA, typename FB>
 __device__ void wg_gemm_tile_fn(int M, int N, int Kd, const FA &fa, const FB &fb, T *C, int64_t ldc, int ti, int tj,
-                                T *lds) {
+                                T *lds, int k_lo = 0, int k_hi = -1) {
     auto epi = [=](int gi, int gj, T v) { C[gi + (int64_t) gj * ldc] = v; };
-    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, true, false, epi, ti, tj, lds);
+    wg_gemm_tile_core<T>(M, N, Kd, fa, fb, true, false, epi, ti, tj, lds, k_lo, k_hi);
 }
 
 // Whole-matrix GEMM by one workgroup (loops over all 64x64 tiles).

@@ -1312,13 +1312,17 @@ __device__ void ph_recon_nov(const RecompDesc<T> &d, T *lds) {
         const T *R2 = w.j;  // R2[i, j] = J[i + j mu] for i <= j
         auto fa = [=](int j, int i) -> T { return i <= j ? R2[i + (int64_t) j * mu] : T(0); };
         auto fb = [=](int i, int c) -> T { return U[i + (int64_t) c * m]; };
-        for (int t = 0; t < tm * tn; ++t) wg_gemm_tile_fn<T>(kt, r, kt, fa, fb, w.vtmp, n, t % tm, t / tm, lds);
+        // row j of R2^T vanishes beyond column j: the K range of output rows [64 ti, 64 ti + 64)
+        for (int t = 0; t < tm * tn; ++t)
+            wg_gemm_tile_fn<T>(kt, r, kt, fa, fb, w.vtmp, n, t % tm, t / tm, lds, 0, (t % tm) * GT_M + GT_M);
     } else {
         const T *Y = w.y;  // X[piv[i], j] = R_Y[j, i] = Y[j + i n] for i >= j
         const int32_t *piv = w.piv;
         auto fa = [=](int j, int i) -> T { return i >= j ? Y[j + (int64_t) i * n] : T(0); };
         auto fb = [=](int i, int c) -> T { return U[piv[i] + (int64_t) c * m]; };
-        for (int t = 0; t < tm * tn; ++t) wg_gemm_tile_fn<T>(kt, r, mu, fa, fb, w.vtmp, n, t % tm, t / tm, lds);
+        // row j of R_Y^T P vanishes before column j: the K range starts at the tile's first row
+        for (int t = 0; t < tm * tn; ++t)
+            wg_gemm_tile_fn<T>(kt, r, mu, fa, fb, w.vtmp, n, t % tm, t / tm, lds, (t % tm) * GT_M);
     }
     __syncthreads();
     HCAMD_SUB_END(t_rf, w.hdr, 6);
