@@ -435,7 +435,9 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         ph_stack<T>(d, 0, 1);
         __syncthreads();
         const Dims z = dims_of(d, w);
-        if (z.K <= d.kmax) ph_features<T>(d, lds);
+        // features for the rank model, and in the exact form, whose runs are its training data;
+        // the tolerance rule without a model reads the core width off R_Y and needs none
+        if (z.K <= d.kmax && (d.rank_model || !(d.model_tol > 0.0))) ph_features<T>(d, lds);
         uint64_t t_qru = 0, t_y = 0, t_qry = 0, t_svd = 0, t_ru = 0;
         if (z.K <= d.kmax) {
             wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds, nullptr, w.hdr);

@@ -53,6 +53,7 @@ __device__ void ph_features(const RecompDesc<T> &d, T *lds) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int n = d.n, ra = d.arank, rc = z.K - d.arank, l = lane_id();
+    const StackedV<T> vs = stacked_v(d, z.K);
     double *G = reinterpret_cast<double *>(lds);  // 2 x 8 x 8
     double *diag = G + 128;                        // 16
     for (int e = wave_id(); e < 72; e += NWAVES) {
@@ -66,8 +67,7 @@ __device__ void ph_features(const RecompDesc<T> &d, T *lds) {
         const int c0 = blk == 0 ? 0 : rc, nc = blk == 0 ? rc : ra;
         double sum = 0.0;
         if (b < nc) {
-            const T *x = w.vs + (int64_t) (c0 + a) * n, *y = w.vs + (int64_t) (c0 + b) * n;
-            for (int r = l; r < n; r += WAVE) sum += (double) x[r] * (double) y[r];
+            for (int r = l; r < n; r += WAVE) sum += (double) vs(r, c0 + a) * (double) vs(r, c0 + b);
         }
         sum = wave_sum(sum);
         if (l == 0) {

@@ -114,7 +114,7 @@ __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax
     const int64_t hdr_elems = (HDR_WORDS * 8 + sizeof(T) - 1) / sizeof(T);
     L.hdr = o; o += align_up(hdr_elems, 32);
     L.us = o; o += align_up(m * kmax, 32);
-    L.vs = o; o += align_up(n * kmax, 32);
+    L.vs = o;  // no region: V_s is read in place from C_V and Bfin (StackedV)
     L.y = o; o += align_up(n * mu, 32);
     L.tau_u = o; o += align_up(kmax, 32);
     L.tau_y = o; o += align_up(kmax, 32);
@@ -136,7 +136,7 @@ __host__ __device__ inline WsLayout ws_layout(int64_t m, int64_t n, int64_t kmax
 template <typename T>
 struct Ws {
     int64_t *hdr;
-    T *us, *vs, *y, *tau_u, *tau_y, *t_u, *t_y, *j, *v1, *sig, *vtmp, *j2, *tau_j, *t_j;
+    T *us, *y, *tau_u, *tau_y, *t_u, *t_y, *j, *v1, *sig, *vtmp, *j2, *tau_j, *t_j;
     int32_t *perm;
     int32_t *piv;  // column permutation of the pivoted QR of Y
 };
@@ -147,7 +147,6 @@ __device__ inline Ws<T> ws_of(const RecompDesc<T> &d) {
     Ws<T> w;
     w.hdr = reinterpret_cast<int64_t *>(d.ws + L.hdr);
     w.us = d.ws + L.us;
-    w.vs = d.ws + L.vs;
     w.y = d.ws + L.y;
     w.tau_u = d.ws + L.tau_u;
     w.tau_y = d.ws + L.tau_y;
@@ -199,19 +198,50 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
         w.hdr[HDR_TRAIL2] = 0;
     }
     if (K > d.kmax) return;
+    (void) n;
     const T *cu = d.cdata;
-    const T *cv = d.cdata + m * d.max_rank;  // ld rc
-    const int64_t totU = m * K, totV = n * K;
-    const int64_t stride = (int64_t) nwg * NT;
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < totU; e += stride) {
-        const int64_t i = e % m, j = e / m;
-        w.us[e] = (j < rc) ? cu[i + j * m] : d.alpha * d.afin[i + (j - rc) * d.ldaf];
+    const T *af = d.afin;
+    const T alpha = d.alpha;
+    const int64_t ldaf = d.ldaf;
+    // U_s only: V_s = [beta C_V^T | Bfin] is read in place by its one consumer, Y = V_s R_U^T
+    // (ph_y).  m * K < 2^31 (m <= 2048, K <= 768), so the element index is 32-bit; four elements
+    // per thread and pass have their loads in flight together.
+    const int mm = (int) m, totU = (int) (m * K), irc = (int) rc;
+    constexpr int EPT = 4;
+    for (int e0 = wg * NT * EPT + (int) threadIdx.x; e0 < totU; e0 += nwg * NT * EPT) {
+        T v[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = e0 + q * NT;
+            if (e < totU) {
+                const int i = e % mm, j = e / mm;
+                v[q] = (j < irc) ? cu[i + (int64_t) j * mm] : alpha * af[i + (int64_t) (j - irc) * ldaf];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+            if (e0 + q * NT < totU) w.us[e0 + q * NT] = v[q];
     }
-    // V_s[j, i] = beta * C_V[i, j] for i < rc ; V_s[j, rc + i] = Bfin[i, j]  (ProcessVpointer)
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < totV; e += stride) {
-        const int64_t j = e % n, i = e / n;
-        w.vs[e] = (i < rc) ? d.beta * cv[i + j * rc] : d.bfin[(i - rc) + j * d.ldbf];
+}
+
+// V_s[i, k] of the stacked right factor (n x K): beta * C_V[k, i] for k < rc, Bfin[k - rc, i]
+// after (ProcessVpointer, reference src/kernels/omp/kernels.cpp:69-76), read in place from C's
+// V factor (rc x n, ld rc) and Bfin (ld ldbf).
+template <typename T>
+struct StackedV {
+    const T *cv, *bf;
+    T beta;
+    int rc;
+    int64_t ldbf;
+    __device__ T operator()(int i, int k) const {
+        return k < rc ? beta * cv[k + (int64_t) i * rc] : bf[(k - rc) + (int64_t) i * ldbf];
     }
+};
+
+template <typename T>
+__device__ inline StackedV<T> stacked_v(const RecompDesc<T> &d, int K) {
+    const int rc = K - d.arank;
+    return StackedV<T>{d.cdata + (int64_t) d.m * d.max_rank, d.bfin, d.beta, rc, d.ldbf};
 }
 
 // Leading dimension of a Householder panel of `rows` rows staged in LDS: odd, so that the 16
@@ -558,15 +588,25 @@ __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, i
         // squared norms over rows [j0, m) by current column position, left by the previous trailing update
         for (int c = threadIdx.x; c < nt; c += NT) nrm[c] = cn[j0 + c];
     } else {
-        for (int c = wave_id(); c < nt; c += NWAVES) {
-            const T *col = A + (int64_t) (j0 + c) * lda;
-            T sum = T(0);
+        // four columns per wave at a time (their loads in flight together), one multi-sum
+        constexpr int CW = 4;
+        for (int cb = wave_id(); cb < nt; cb += CW * NWAVES) {
+            T sum[CW];
+#pragma unroll
+            for (int u = 0; u < CW; ++u) sum[u] = T(0);
             for (int r = j0 + l; r < m; r += WAVE) {
-                const T v = col[r];
-                sum += v * v;
+#pragma unroll
+                for (int u = 0; u < CW; ++u) {
+                    const int c = cb + u * NWAVES;
+                    if (c < nt) {
+                        const T v = A[r + (int64_t) (j0 + c) * lda];
+                        sum[u] += v * v;
+                    }
+                }
             }
-            sum = wave_sum(sum);
-            if (l == 0) nrm[c] = sum;
+            const T tot = wave_multi_sum_fast<T, CW>(sum);
+            const int u = wave_multi_sum_index<CW>(l), c = cb + u * NWAVES;
+            if (l == multi_sum_src_lane<CW>(u) && c < nt) nrm[c] = tot;
         }
     }
     for (int c = threadIdx.x; c < nt; c += NT) {
@@ -590,9 +630,7 @@ __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, i
     }
     __syncthreads();
     // The jb swaps (t, where[sel[t]]) in turn define the permutation; afterwards at[q] is the
-    // position (relative to j0) whose column ends at q.  The positions that changed and their
-    // sources go to the move list mv (pairs destination, source; count in mv[4 PB]), so the
-    // columns move in one pass over the rows.
+    // position (relative to j0) whose column ends at q.  Only thread 0 runs the swaps, on LDS.
     if (threadIdx.x == 0) {
         for (int t = 0; t < jb; ++t) {
             const int c = sel[t], p = where[c];
@@ -603,24 +641,36 @@ __device__ T window_pivot(int m, int ncols, T *A, int64_t lda, int j0, int jb, i
                 at[p] = c2;
                 where[c] = t;
                 where[c2] = p;
-                const int32_t x = piv[j0 + t];
-                piv[j0 + t] = piv[j0 + p];
-                piv[j0 + p] = x;
             }
         }
-        int nmv = 0;
-        for (int i = 0; i < 2 * jb; ++i) {
-            const int q = i < jb ? i : sel[PB + i - jb];
-            if (at[q] == q) continue;
-            bool seen = false;
-            for (int k = 0; k < nmv; ++k) seen |= mv[2 * k] == q;
-            if (!seen) {
-                mv[2 * nmv] = q;
-                mv[2 * nmv + 1] = at[q];
-                ++nmv;
+    }
+    __syncthreads();
+    // The positions that changed lie among 0..jb-1 and the swap sources sel[PB..PB+jb); wave 0
+    // takes one candidate per lane (2 jb <= 64), keeps each changed position once, and writes the
+    // move list mv (pairs destination, source; count in mv[4 PB]), so the columns then move in
+    // one pass over the rows.  The pivot vector follows: piv[j0 + q] <- piv[j0 + at[q]] (every
+    // lane's load completes before any lane's store, one instruction each).
+    if (wave_id() == 0) {
+        int q = 0;
+        bool keep = false;
+        if (l < 2 * jb) {
+            q = l < jb ? l : sel[PB + l - jb];
+            keep = at[q] != q;
+            if (keep && l >= jb) {
+                if (q < jb) keep = false;
+                for (int k = 0; k < l - jb; ++k) keep = keep && sel[PB + k] != q;
             }
         }
-        mv[4 * PB] = nmv;
+        const unsigned long long bal = __ballot(keep);
+        const int src = keep ? at[q] : 0;
+        const int32_t pv = keep ? piv[j0 + src] : 0;
+        if (keep) {
+            const int k = __popcll(bal & ((1ull << l) - 1ull));
+            mv[2 * k] = q;
+            mv[2 * k + 1] = src;
+            piv[j0 + q] = pv;
+        }
+        if (l == 0) mv[4 * PB] = __popcll(bal);
     }
     __syncthreads();
     const int nmv = mv[4 * PB];
@@ -924,9 +974,17 @@ __device__ void ph_y(const RecompDesc<T> &d, int wg, int nwg, T *lds) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int tm = (d.n + GT_M - 1) / GT_M, tn = (z.mu + GT_N - 1) / GT_N;
+    const StackedV<T> fa = stacked_v(d, z.K);  // read K-fastest: consecutive addresses in C_V and Bfin
+    const T *us = w.us;
+    const int64_t ldu = d.m;
+    auto fb = [=](int k, int j) -> T { return k < j ? T(0) : us[j + (int64_t) k * ldu]; };  // R_U^T
+    T *y = w.y;
+    const int64_t ldy = d.n;
+    auto epi = [=](int gi, int gj, T v) { y[gi + (int64_t) gj * ldy] = v; };
+    // R_U^T(k, j) vanishes for k < j: the tile's K range starts at its first column
     for (int t = wg; t < tm * tn; t += nwg)
-        wg_gemm_tile<T>(d.n, z.mu, z.K, T(1), w.vs, d.n, false, w.us, d.m, true, T(0), w.y, d.n, t % tm, t / tm,
-                        lds, /*upper_b=*/true);
+        wg_gemm_tile_core<T>(d.n, z.mu, z.K, fa, fb, /*a_rows_fast=*/false, /*b_cols_fast=*/true, epi, t % tm,
+                             t / tm, lds, (t / tm) * GT_N);
 }
 
 // Phase (single workgroup): truncate the core and set up the Jacobi SVD.

@@ -294,7 +294,8 @@ hcorepp_status hcorepp_tile_syrk_s(hcorepp_ctx *ctx, float alpha, const hcorepp_
 /* ---------------------------------------------------------------------------
  * Learned rank predictor (no counterpart in the reference).  Per-update features are computed on
  * the device right after stacking (hcorepp_batch_features copies them, RANK_FEATURES floats per
- * update, after a pass); an MLP of hcorepp_rank_model_params() parameters predicts log2 of the
+ * update, after a pass with a model set or in the exact form -- the tolerance rule without a
+ * model skips them and leaves zeros); an MLP of hcorepp_rank_model_params() parameters predicts log2 of the
  * post-truncation rank.  With a model set (BF16 parameters in device memory, from
  * hcorepp_mlp_pack_bf16), each update's core is presized to the prediction + margin columns and
  * accepted only if the dropped rows of the pivoted R factor have Frobenius norm <= tol * threshold;
