@@ -356,6 +356,116 @@ __device__ __forceinline__ int wg_block_jacobi(T *J, int rows, int ncols, int64_
     return sweep;
 }
 
+// One tile (block pair) of a staged super-block pair, in LDS (ld rows, 2 W columns).
+template <typename T, int RPL, int TW>
+__device__ __forceinline__ int staged_tile(T *Jl, int rows, int lcols, int lI, int lJ, bool full, T tol, T floor2) {
+    T y[RPL][TW], vt[vrows<0>()][TW];
+    jtile_load<T, RPL, 0, TW>(y, vt, Jl, rows, lcols, rows, nullptr, 0, lI, lJ);
+    const int a = jtile_rounds<T, RPL, 0, TW>(y, vt, full, tol, floor2);
+    if (a & JROT_APPLIED) jtile_store<T, RPL, 0, TW>(y, vt, Jl, rows, lcols, rows, nullptr, 0, lI, lJ);
+    return a;
+}
+
+// Block Jacobi without V for a core too large for LDS, staged through it by super-blocks.  The
+// columns form super-blocks of W (a multiple of B = TW / 2) and each sweep runs the round-robin
+// ordering over super-blocks.  For a super-block pair (SI, SJ), both are copied into the LDS
+// buffer Jl (ld rows; SI in local columns [0, W), SJ in [W, 2 W), columns past ncols zero, so they
+// never rotate), the four waves rotate its block pairs there round by round, and the pair is
+// copied back.  In the first super-step every super-block appears once, and its pair runs the
+// round-robin over all 2 W / B local blocks (in whose first round each block appears once, as a
+// full tile: its internal column pairs); later super-steps visit only the block pairs across SI
+// and SJ, W / B rounds.  So every column pair is visited once per sweep, as in wg_block_jacobi,
+// while the core passes through memory once per super-step instead of once per block step:
+// W / B times fewer passes.  J: rows x ncols (ld rows, rows <= 64 RPL); Jl: 2 W rows elements of
+// LDS; gbuf: block_jacobi_lds_elems<T, TW>() elements of LDS.  Returns the sweeps used.
+template <typename T, int RPL, int TW>
+__device__ __forceinline__ int wg_staged_jacobi(T *J, int rows, int ncols, T *Jl, int W, int max_sweeps, T *gbuf) {
+    constexpr int B = TW / 2;
+    T *s_red = gbuf;
+    int &s_rot = *reinterpret_cast<int *>(s_red + NWAVES);
+    const int64_t ne = (int64_t) rows * ncols;
+    T part = T(0);
+    for (int64_t e = threadIdx.x; e < ne; e += NT) {
+        const T a = J[e];
+        part += a * a;
+    }
+    const T fro2 = wg_sum(part, s_red);
+    const T floor2 = Eps<T>::value * Eps<T>::value * fro2;
+    const T tol = sqrt(T(rows)) * Eps<T>::value;
+    if (ncols <= 1) return 0;
+    const int a = W / B, lcols = 2 * W;
+    const int ns = (ncols + W - 1) / W, nsp = ns + (ns & 1);
+    const int64_t sw = (int64_t) W * rows;
+    int sweep = 0;
+    for (; sweep < max_sweeps; ++sweep) {
+        if (threadIdx.x == 0) s_rot = 0;
+        __syncthreads();
+        int any = 0;
+        for (int ss = 0; ss < nsp - 1; ++ss) {
+            for (int sp = 0; sp < nsp / 2; ++sp) {
+                int cI, cJ;
+                if (!jacobi_pair_cols(sp, ss, nsp, W, ncols, cI, cJ)) continue;
+                const bool solo = cJ >= ncols;  // paired with the padding super-block
+                if (ss > 0 && solo) continue;
+                const int64_t nI = (int64_t) ((ncols - cI) < W ? (ncols - cI) : W) * rows;
+                const int64_t nJ = solo ? 0 : (int64_t) ((ncols - cJ) < W ? (ncols - cJ) : W) * rows;
+                wg_copy(Jl, J + (int64_t) cI * rows, nI);
+                if (nJ > 0) wg_copy(Jl + sw, J + (int64_t) cJ * rows, nJ);
+                for (int64_t e = nI + threadIdx.x; e < sw; e += NT) Jl[e] = T(0);
+                for (int64_t e = nJ + threadIdx.x; e < sw; e += NT) Jl[sw + e] = T(0);
+                __syncthreads();
+                // local block k holds real columns when its first global column is below ncols;
+                // a tile is skipped when no pair of it can rotate (a cross tile with an empty
+                // block, a full tile with two)
+                const int realI = (int) ((nI / rows + B - 1) / B), realJ = (int) ((nJ / rows + B - 1) / B);
+                auto real = [&](int lc) { return lc < W ? lc / B < realI : (lc - W) / B < realJ; };
+                if (ss == 0) {
+                    // the round-robin over the local blocks: all 2 a of them, or the a of SI alone
+                    const int nb = solo ? a : 2 * a;
+                    for (int t = 0; t < nb - 1; ++t) {
+                        for (int p = wave_id(); p < nb / 2; p += NWAVES) {
+                            int lI, lJ;
+                            jacobi_pair_cols(p, t, nb, B, lcols, lI, lJ);
+                            const bool full = t == 0;
+                            if (full ? (real(lI) || real(lJ)) : (real(lI) && real(lJ)))
+                                any |= staged_tile<T, RPL, TW>(Jl, rows, lcols, lI, lJ, full, tol, floor2);
+                        }
+                        __syncthreads();
+                    }
+                } else {
+                    for (int t = 0; t < a; ++t) {
+                        for (int p = wave_id(); p < a; p += NWAVES) {
+                            int q = p + t;
+                            if (q >= a) q -= a;
+                            if (real(p * B) && real(W + q * B))
+                                any |= staged_tile<T, RPL, TW>(Jl, rows, lcols, p * B, W + q * B, false, tol, floor2);
+                        }
+                        __syncthreads();
+                    }
+                }
+                wg_copy(J + (int64_t) cI * rows, Jl, nI);
+                if (nJ > 0) wg_copy(J + (int64_t) cJ * rows, Jl + sw, nJ);
+                __syncthreads();
+            }
+        }
+        if ((any & JROT_LARGE) && lane_id() == 0) atomicOr(&s_rot, 1);
+        __syncthreads();
+        const int rot = s_rot;
+        __syncthreads();
+        if (!rot) break;
+    }
+    return sweep;
+}
+
+// Super-block width of wg_staged_jacobi for `rows` rows, blocks of B columns and an LDS buffer of
+// `cap` elements: the widest with two super-blocks in the buffer whose block count is a multiple
+// of the wave count (every inner round then gives each wave the same number of tiles); 0 when
+// not even NWAVES blocks fit.
+__host__ __device__ inline int staged_width(int rows, int B, int64_t cap) {
+    const int64_t a = cap / (2 * (int64_t) (rows > 0 ? rows : 1)) / B / NWAVES * NWAVES;
+    return a >= NWAVES ? (int) (a * B) : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Workgroup-cooperative variant: the four waves work on the same tile of TW columns, each
 // owning an interleaved quarter of the rows of J (row lane + 64 (wave + 4 t), t < RPL) and of V

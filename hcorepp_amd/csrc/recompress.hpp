@@ -1257,6 +1257,7 @@ __device__ void ph_svd(const RecompDesc<T> &d, T *lds) {
 // ---------------------------------------------------------------------------
 constexpr int64_t BATCH_LDS_BYTES = 78 * 1024;  // dynamic LDS per workgroup of the batched kernel
 constexpr int NOV_GBUF = 16;                     // reduction slots ahead of the LDS-resident core
+static_assert(NOV_GBUF >= block_jacobi_lds_elems<double, 16>(), "the block Jacobi's slots fit before the core");
 
 template <typename T>
 __host__ __device__ constexpr int64_t nov_lds_cap() {
@@ -1281,9 +1282,20 @@ __device__ HCAMD_PHASE_ATTR int jacobi_nov_nl(T *J, int rows, int ncols, T *gbuf
     return wg_block_jacobi<T, RPL, 0, TW, (RPL <= 2)>(J, rows, ncols, rows, nullptr, 0, JACOBI_MAX_SWEEPS, gbuf);
 }
 
+// Core in global memory staged through LDS by super-blocks of W columns (wg_staged_jacobi): the
+// buffer after the reduction slots holds two super-blocks.
+template <typename T, int RPL, int TW>
+__device__ HCAMD_PHASE_ATTR int jacobi_nov_staged_nl(T *J, int rows, int ncols, int W) {
+    extern __shared__ unsigned char smem_raw[];
+    T *gbuf = reinterpret_cast<T *>(smem_raw);
+    return wg_staged_jacobi<T, RPL, TW>(J, rows, ncols, gbuf + NOV_GBUF, W, JACOBI_MAX_SWEEPS, gbuf);
+}
+
 template <typename T, int RPL, int TW>
 __device__ int jacobi_nov_shape(T *J, int rows, int ncols, T *lds, bool resident) {
-    return resident ? jacobi_nov_lds_nl<T, RPL, TW>(rows, ncols) : jacobi_nov_nl<T, RPL, TW>(J, rows, ncols, lds);
+    if (resident) return jacobi_nov_lds_nl<T, RPL, TW>(rows, ncols);
+    const int W = staged_width(rows, TW / 2, nov_lds_cap<T>());
+    return W > 0 ? jacobi_nov_staged_nl<T, RPL, TW>(J, rows, ncols, W) : jacobi_nov_nl<T, RPL, TW>(J, rows, ncols, lds);
 }
 
 // One-sided Jacobi of J (rows x ncols, ld rows) in place, without V; `lds` is the kernel's
