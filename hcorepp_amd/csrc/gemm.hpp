@@ -1,8 +1,8 @@
 // Workgroup-level tiled GEMM for the factor products and dense tile updates
 // (HCoreKernels::Gemm call sites: reference src/api/HCore.cpp:176-264 via
 // src/operators/concrete/Dense.cpp:46-108).  One workgroup computes one
-// 64x64 tile of C = alpha*op(A)*op(B) + beta*C with a 16-deep K chunk staged
-// in LDS; each thread owns a 4x4 register block.
+// 64x64 tile of C = alpha*op(A)*op(B) + beta*C with a K chunk (32 deep in FP64, 16 in FP32)
+// staged in LDS; FP64 on the matrix cores, FP32 with a 4x4 register block per thread.
 #pragma once
 
 #include "device_common.hpp"
@@ -10,10 +10,18 @@
 namespace hcamd {
 
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
-constexpr int GT_LDS_ELEMS = GT_K * (GT_M + 1) + GT_K * (GT_N + 1);
+// K chunk of the FP64 (matrix-core) form: 32 deep, so that each thread has 16 loads in flight
+// per chunk and a tile takes half as many memory round trips (the FP32 form and the mixed split
+// keep 16: the split's exactness bound counts 16-deep sums).
+constexpr int GT_KD = 32;
+constexpr int GT_LDS_ELEMS = GT_KD * (GT_M + 1) + GT_KD * (GT_N + 1);
+template <typename T>
+__host__ __device__ constexpr int gemm_kc() {
+    return sizeof(T) == 8 ? GT_KD : GT_K;
+}
 
-// The K loop is software-pipelined: the next chunk's elements are loaded into registers (four of
-// A and four of B per thread) right after the current chunk is in LDS, so their memory latency
+// The K loop (chunks of gemm_kc<T>() = KC) is software-pipelined: the next chunk's elements are
+// loaded into registers (KC / 4 of A and of B per thread) right after the current chunk is in LDS, so their memory latency
 // overlaps the chunk's arithmetic.  `fa(i, k)` and `fb(k, j)` return op(A)[i, k] and op(B)[k, j]
 // (zero outside the operands); A's chunk is read row-fastest when `a_rows_fast`, B's K-fastest
 // unless `b_cols_fast`, so that consecutive threads read consecutive addresses.  `epi(i, j, v)`
@@ -29,20 +37,21 @@ template <typename T, typename FA, typename FB, typename EPI>
 __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA &fa, const FB &fb, bool a_rows_fast,
                                                   bool b_cols_fast, const EPI &epi, int ti, int tj, T *lds,
                                                   int k_lo = 0, int k_hi = -1) {
-    k_lo = (k_lo / GT_K) * GT_K;
+    constexpr int KC = gemm_kc<T>();
+    k_lo = (k_lo / KC) * KC;
     if (k_hi < 0 || k_hi > Kd) k_hi = Kd;
-    T *As = lds;                      // [GT_K][GT_M + 1]  As[k][i]
-    T *Bs = lds + GT_K * (GT_M + 1);  // [GT_K][GT_N + 1]  Bs[k][j]
+    T *As = lds;                    // [KC][GT_M + 1]  As[k][i]
+    T *Bs = lds + KC * (GT_M + 1);  // [KC][GT_N + 1]  Bs[k][j]
     const int i0 = ti * GT_M, j0 = tj * GT_N;
-    constexpr int PER = GT_M * GT_K / NT;  // elements of each operand chunk per thread (4)
-    static_assert(GT_M * GT_K == GT_N * GT_K && PER * NT == GT_M * GT_K, "tile shape");
+    constexpr int PER = GT_M * KC / NT;  // elements of each operand chunk per thread
+    static_assert(GT_M * KC == GT_N * KC && PER * NT == GT_M * KC, "tile shape");
     auto a_pos = [&](int e, int &i, int &k) {
         if (a_rows_fast) {
             i = e % GT_M;
             k = e / GT_M;
         } else {
-            k = e % GT_K;
-            i = e / GT_K;
+            k = e % KC;
+            i = e / KC;
         }
     };
     auto b_pos = [&](int e, int &j, int &k) {
@@ -50,8 +59,8 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
             j = e % GT_N;
             k = e / GT_N;
         } else {
-            k = e % GT_K;
-            j = e / GT_K;
+            k = e % KC;
+            j = e / KC;
         }
     };
     T ra[PER], rb[PER];
@@ -87,12 +96,12 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = hcamd_doublex4{0.0, 0.0, 0.0, 0.0};
         if (k_lo < k_hi) load(k_lo);
-        for (int k0 = k_lo; k0 < k_hi; k0 += GT_K) {
+        for (int k0 = k_lo; k0 < k_hi; k0 += KC) {
             stage();
             __syncthreads();
-            if (k0 + GT_K < k_hi) load(k0 + GT_K);
+            if (k0 + KC < k_hi) load(k0 + KC);
 #pragma unroll
-            for (int s = 0; s < GT_K / 4; ++s) {
+            for (int s = 0; s < KC / 4; ++s) {
                 const int kr = 4 * s + k4;
                 double av[2], bv[2];
 #pragma unroll
@@ -124,12 +133,12 @@ __device__ __forceinline__ void wg_gemm_tile_core(int M, int N, int Kd, const FA
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
         if (k_lo < k_hi) load(k_lo);
-        for (int k0 = k_lo; k0 < k_hi; k0 += GT_K) {
+        for (int k0 = k_lo; k0 < k_hi; k0 += KC) {
             stage();
             __syncthreads();
-            if (k0 + GT_K < k_hi) load(k0 + GT_K);
+            if (k0 + KC < k_hi) load(k0 + KC);
 #pragma unroll
-            for (int k = 0; k < GT_K; ++k) {
+            for (int k = 0; k < KC; ++k) {
                 T av[4], bv[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -171,8 +180,8 @@ __device__ void wg_gemm_tile(int M, int N, int Kd, T alpha, const T *A, int64_t 
 }
 
 // One 64 x 64 tile of C (M x N, ld ldc) = sum_k fa(i, k) fb(k, j) for operands given as element
-// functions (each returns zero outside its structure: triangles, gathered rows); 16-deep K
-// chunks are staged in LDS as in wg_gemm_tile.  A's chunk is read row-fastest, B's K-fastest.
+// functions (each returns zero outside its structure: triangles, gathered rows); K chunks are
+// staged in LDS as in wg_gemm_tile.  A's chunk is read row-fastest, B's K-fastest.
 template <typename T, typename FA, typename FB>
 __device__ void wg_gemm_tile_fn(int M, int N, int Kd, const FA &fa, const FB &fb, T *C, int64_t ldc, int ti, int tj,
                                 T *lds, int k_lo = 0, int k_hi = -1) {
