@@ -273,6 +273,41 @@ def test_batched_matches_plain(ctx):
         assert rel_fro(Cs_b[i].to_dense(), Cs_p[i].to_dense()) < 1e-12
 
 
+def test_batched_wide_cores_match_oracle(ctx):
+    """Cores too large for the workgroup's LDS (more than 9968 elements: kt >= 100 here), whose
+    Jacobi is staged through LDS by super-block pairs, in the exact and the rule form against the
+    CPU oracle: ranks equal (up to rounding ties at the threshold), exact form within 1e-10 and
+    rule form within the accuracy (1e-8) of the oracle's result."""
+    H, O = _H(), _orc()
+    from hcorepp_amd.workloads import batch_ccc
+    ups = batch_ccc(12, 512, seed=61, lo=120, hi=192)
+    acc = 1e-8
+    want = []
+    for u in ups:
+        oC = O.Tile.compressed(u["cu"], u["cv"], max_rank=384)
+        _, sig = O.hcore_gemm(1.0, O.Tile.compressed(u["au"], u["av"]), O.Tile.compressed(u["bu"], u["bv"]), 1.0,
+                              oC, O.Params(acc), want_sigma=True)
+        want.append((oC, sig))
+    for tol, bound in ((0.0, 1e-10), (0.1, acc)):
+        As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
+        Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
+        Cs = [H.CompressedTile(u["cu"], u["cv"], max_rank=384) for u in ups]
+        b = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, H.CompressionParameters(acc))
+        b.set_rank_model(None, tol=tol, margin=0)
+        b.run()
+        rk = b.sync_ranks_to_tiles()
+        assert b.status()[0] == 0
+        kw = b.core_widths()
+        print(f"tol {tol}: core widths {int(kw.min())}-{int(kw.max())}, ranks {int(rk.min())}-{int(rk.max())}")
+        assert int(kw.max()) >= 100  # the staged path ran
+        for i, (oC, sig) in enumerate(want):
+            if rk[i] != oC.rank:
+                assert rank_tie(sig, oC.rank, rk[i], acc), (tol, i, rk[i], oC.rank)
+                continue
+            assert rel_fro(Cs[i].to_dense(), oC.to_dense()) < bound, (tol, i)
+        b.close()
+
+
 def test_batched_mixed_precision_meets_1e8(ctx):
     """Mixed form: the factor products on FP32 matrix cores with FP32 accumulation, over the
     operands' error-free slices (10-bit integer slices per power-of-two row/column scale, exact
