@@ -112,41 +112,59 @@ __host__ __device__ __forceinline__ constexpr int jslot_col(int rd, int slot) {
     return slot < TW / 2 ? slot : TW / 2 + (slot - TW / 2 + rd) % (TW / 2);
 }
 
+// The value a lane's group partner holds in the other half of its group of G = 64 / NP lanes
+// (groups of 8: half-row mirror, 16: row mirror, 32: the odd rows by a permlane16 swap, 64: the
+// upper half-wave by a permlane32 swap); lanes in the lower half of a group read the upper half.
+template <typename T, int NP>
+__device__ __forceinline__ T group_partner(T v) {
+    constexpr int G = WAVE / NP;
+    if constexpr (G == 8) {
+        return dpp_update<DPP_ROW_HALF_MIRROR, 0xF, 0xF>(v, v);
+    } else if constexpr (G == 16) {
+        return dpp_update<DPP_ROW_MIRROR, 0xF, 0xF>(v, v);
+    } else {
+        static_assert(G == 32 || G == 64, "pair groups of 8 to 64 lanes");
+        T a = v, b = v;
+        permlane_swap<G / 2>(a, b);
+        return b;
+    }
+}
+
 template <typename T, int RPL, int RPLV, int TW, bool FULL>
 __device__ __forceinline__ int jacobi_round_fixed(T (&y)[RPL][TW], T (&vt)[vrows<RPLV>()][TW], T tol, T floor2) {
-    constexpr int NP = TW / 2;
+    constexpr int NP = TW / 2, G = WAVE / NP;
     const int l = lane_id();
-    // the squared norms (2 NP values) and the cross products (NP values) are reduced separately:
-    // both counts are powers of two, so no padded slot takes part in the exchanges
+    // the squared norms (2 NP values, pair k's at 2k and 2k + 1) and the cross products (NP values)
+    // are reduced separately: both counts are powers of two, so no padded slot takes part.  After
+    // the reductions the G-lane group of pair k holds its cross product, the group's lower half
+    // its first column's squared norm and the upper half its second's: the group's first lane
+    // has all three after one exchange within the group, and computes the pair's rotation.
     T nrm[2 * NP], crs[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int q = jslot_q<T, TW, FULL>(k);
-        nrm[k] = nrm[NP + k] = crs[k] = T(0);
+        nrm[2 * k] = nrm[2 * k + 1] = crs[k] = T(0);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const T a = y[t][k], b = y[t][q];
-            nrm[k] += a * a;
-            nrm[NP + k] += b * b;
+            nrm[2 * k] += a * a;
+            nrm[2 * k + 1] += b * b;
             crs[k] += a * b;
         }
     }
-    const T tn = wave_multi_sum_fast<T, 2 * NP>(nrm);
-    const T tc = wave_multi_sum_fast<T, NP>(crs);
-    const int kk = l < NP ? l : 0;
-    const T app = __shfl(tn, multi_sum_src_lane<2 * NP>(kk), WAVE);
-    const T aqq = __shfl(tn, multi_sum_src_lane<2 * NP>(NP + kk), WAVE);
-    const T apq = __shfl(tc, multi_sum_src_lane<NP>(kk), WAVE);
+    const T app = wave_multi_sum_fast<T, 2 * NP>(nrm);
+    const T apq = wave_multi_sum_fast<T, NP>(crs);
+    const T aqq = group_partner<T, NP>(app);
     T c, s;
     int lg;
     jacobi_rotation<T, (RPLV > 0)>(app, aqq, apq, tol, floor2, c, s, lg);
-    const unsigned long long lgm = __ballot(l < NP && lg);
+    const unsigned long long lgm = __ballot((l % G) == 0 && lg);
     int any = 0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const T sk = read_lane(s, k);
+        const T sk = read_lane(s, k * G);
         if (sk != T(0)) {
-            const T ck = read_lane(c, k);
+            const T ck = read_lane(c, k * G);
             const int q = jslot_q<T, TW, FULL>(k);
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
@@ -160,7 +178,7 @@ __device__ __forceinline__ int jacobi_round_fixed(T (&y)[RPL][TW], T (&vt)[vrows
                 vt[r][k] = ck * va - sk * vb;
                 vt[r][q] = sk * va + ck * vb;
             }
-            any |= JROT_APPLIED | (((lgm >> k) & 1ull) ? JROT_LARGE : 0);
+            any |= JROT_APPLIED | (((lgm >> (k * G)) & 1ull) ? JROT_LARGE : 0);
         }
     }
     return any;
