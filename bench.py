@@ -38,6 +38,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Hardware queues of this process: HIP maps streams onto GPU_MAX_HW_QUEUES queues (its default is
+# 4), so the concurrent plain leg's PLAIN_STREAMS contexts only run side by side with at least as
+# many queues.  Read once when the HIP runtime starts, hence set before anything initialises it;
+# a larger value from the caller is kept.
+HW_QUEUES = 16
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < HW_QUEUES:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 NB, N_UPDATES, LO, HI, DECAY, MAX_RANK, ACC, SEED = 512, 4096, 8, 192, 0.85, 384, 1e-8, 20261204
 LEARNED_TOL, LEARNED_MARGIN = 0.1, 40  # accept a presized core when the dropped part <= 0.1 * accuracy
 PEAK_FP64_TFLOPS = 78.6
@@ -82,7 +90,7 @@ def alg_bytes_kind(kind, nb, ra, rb, rc, r):
 TLR_N, TLR_NB, TLR_ELL, TLR_ACC, TLR_SEED = 16384, 1024, 0.2, 1e-10, 20261203
 # C tiles of the nb=1024 TLR GEMM sample: a 4 x 4 block of off-diagonal tiles, 16 updates each
 TLR_PICKS = [(i, j) for i in range(4) for j in range(8, 12)]
-PLAIN_STREAMS = 4  # host threads / contexts of the concurrent plain leg (4 hardware queues per process)
+PLAIN_STREAMS = 8  # host threads / contexts (streams) of the concurrent plain leg
 
 
 def host_info():
@@ -195,6 +203,7 @@ def plain_tlr_leg(H, ctx, M, picks):
     tf = flops / dt / 1e12
     out = {"updates_per_s": round(n_up / dt, 3), "updates": n_up, "seconds": round(dt, 3),
            f"updates_per_s_{PLAIN_STREAMS}_streams": round(n_up / dt_streams, 3),
+           "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
            "streams_results_equal_one_stream": bool(same),
            "fp64_tflops": round(tf, 4), "fp64_frac_of_peak": round(tf / PEAK_FP64_TFLOPS, 5),
            "alg_hbm_gb_per_s": round(byts / dt / 1e9, 2), "ranks_min_max": [int(min(ranks_seen)), int(max(ranks_seen))],
