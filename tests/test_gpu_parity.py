@@ -308,6 +308,45 @@ def test_batched_wide_cores_match_oracle(ctx):
         b.close()
 
 
+@pytest.mark.parametrize("ra,rc", [(120, 120), (180, 180), (240, 240)])
+def test_batched_tall_cores_match_oracle(ctx, ra, rc):
+    """Slowly decaying spectra keep the whole stacked rank K = ra + rc in the core (no second QR:
+    the core is K x K), so the batched Jacobi runs on 240, 360 and 480 rows: 4, 6 and 8 rows per
+    lane, staged through LDS by 16- and 8-column super-blocks (240 and 480 rows) or streamed from
+    memory (360 rows).  Exact form against the CPU oracle: ranks equal up to rounding ties, results
+    within 1e-10."""
+    H, O = _H(), _orc()
+    from hcorepp_amd.workloads import lowrank_factors
+    print(f"[case] ra={ra} rc={rc}", flush=True)
+    rng = np.random.default_rng(500 + ra)
+    ups = []
+    for _ in range(2):
+        au, av = lowrank_factors(rng, 512, 512, ra, 0.995)
+        bu, bv = lowrank_factors(rng, 512, 512, ra, 0.995)
+        cu, cv = lowrank_factors(rng, 512, 512, rc, 0.995)
+        ups.append(dict(au=au, av=av, bu=bu, bv=bv, cu=cu, cv=cv))
+    acc = 1e-8
+    As = [H.CompressedTile(u["au"], u["av"]) for u in ups]
+    Bs = [H.CompressedTile(u["bu"], u["bv"]) for u in ups]
+    Cs = [H.CompressedTile(u["cu"], u["cv"], max_rank=384) for u in ups]
+    b = H.Batch(ctx, As, Bs, Cs, 1.0, 1.0, H.CompressionParameters(acc))
+    b.run()
+    rk = b.sync_ranks_to_tiles()
+    assert b.status()[0] == 0
+    kw = b.core_widths()
+    print(f"core widths {kw.tolist()}, ranks {rk.tolist()}")
+    assert int(kw.min()) > 0.9 * (ra + rc)  # the whole stacked rank stays in the core
+    for i, u in enumerate(ups):
+        oC = O.Tile.compressed(u["cu"], u["cv"], max_rank=384)
+        _, sig = O.hcore_gemm(1.0, O.Tile.compressed(u["au"], u["av"]), O.Tile.compressed(u["bu"], u["bv"]), 1.0,
+                              oC, O.Params(acc), want_sigma=True)
+        if rk[i] != oC.rank:
+            assert rank_tie(sig, oC.rank, rk[i], acc), (i, rk[i], oC.rank)
+            continue
+        assert rel_fro(Cs[i].to_dense(), oC.to_dense()) < 1e-10, i
+    b.close()
+
+
 def test_batched_mixed_precision_meets_1e8(ctx):
     """Mixed form: the factor products on FP32 matrix cores with FP32 accumulation, over the
     operands' error-free slices (10-bit integer slices per power-of-two row/column scale, exact
