@@ -359,10 +359,13 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, i
 // in steps 0..3; in the update, MFMA output row i is row r0 + 4 (i % 4) + i / 4, which puts
 // rows r0 + 4 k .. r0 + 4 k + 3 of column j in lane (j, k).  Each lane thus reads and writes 4
 // consecutive elements of its column per group.  P: panel in LDS (rows x 16, ld pld, unit
-// lower); Tm: 16 x 16 upper (LDS, zero below the diagonal).
+// lower); Tm: 16 x 16 upper (LDS, zero below the diagonal).  With `cn`, the squared norms of the
+// updated columns over rows [nrm_lo, rows) go to cn[column] as the update writes them (the next
+// panel's pivoting reads them instead of passing over the trailing matrix once more).
 // ---------------------------------------------------------------------------
 __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const double *Tm, int rows, double *B,
-                                               int64_t ldb, int c0, int c1, bool trans) {
+                                               int64_t ldb, int c0, int c1, bool trans, double *cn = nullptr,
+                                               int nrm_lo = 0) {
     const int l = lane_id(), i16 = l & 15, k4 = l >> 4;
     const int nblk = (c1 - c0 + 15) / 16;
     // A operand of W' = op(T) W, step s: op(T)[i16][4 s + k4]
@@ -418,6 +421,7 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
         for (int s = 0; s < 4; ++s) wp = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[s], w[s], wp, 0, 0, 0);
         // wp[s] = W'[4 s + k4][i16], the B operand of step s of the update B_blk -= P W'; in
         // 64-row chunks the 16 loads of B and of P are in flight together
+        double nsq = 0.0;
         int g0 = 0;
         for (; g0 < r64; g0 += 64) {
             double old[4][4], av[4][4];
@@ -440,7 +444,12 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
 #pragma unroll
                 for (int g = 0; g < 4; ++g)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) bcol[g0 + 16 * g + 4 * k4 + v] = old[g][v] - d[g][v];
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = g0 + 16 * g + 4 * k4 + v;
+                        const double nv = old[g][v] - d[g][v];
+                        bcol[r] = nv;
+                        if (r >= nrm_lo) nsq += nv * nv;
+                    }
             }
         }
         for (; g0 < rows; g0 += 16) {
@@ -457,19 +466,38 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
             if (cok) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    if (rq + v < rows) bcol[rq + v] = old[v] - d[v];
+                    if (rq + v < rows) {
+                        const double nv = old[v] - d[v];
+                        bcol[rq + v] = nv;
+                        if (rq + v >= nrm_lo) nsq += nv * nv;
+                    }
             }
+        }
+        if (cn) {
+            // a column's rows sit in lanes i16 + 16 k4: sum over k4 (lane bits 4 and 5)
+            double a = nsq, b = nsq;
+            permlane_swap<32>(a, b);
+            nsq = a + b;
+            a = nsq;
+            b = nsq;
+            permlane_swap<16>(a, b);
+            nsq = a + b;
+            if (k4 == 0 && cok) cn[col] = nsq;
         }
     }
 }
 
+// `cn` (FP64 16-column panels only: apply_wy_mfma) receives the updated columns' squared norms
+// over rows [nrm_lo, rows).
 template <typename T, int PB>
 __device__ void apply_wy_dispatch(const T *P, int pld, const T *Tm, int rows, T *B, int64_t ldb, int c0, int c1,
-                                  bool trans, T *red) {
+                                  bool trans, T *red, T *cn = nullptr, int nrm_lo = 0) {
     if constexpr (PB == 16 && sizeof(T) == 8) {
-        apply_wy_mfma(P, pld, Tm, rows, B, ldb, c0, c1, trans);
+        apply_wy_mfma(P, pld, Tm, rows, B, ldb, c0, c1, trans, cn, nrm_lo);
         return;
     }
+    (void) cn;
+    (void) nrm_lo;
     // rows <= 512 for PB 16, <= 1024 for PB 8, <= 2048 for PB 4 (pick_pb): at most 32 panel values per lane
     const int chunks = (rows + WAVE * NWAVES - 1) / (WAVE * NWAVES);
     if (chunks <= 1) apply_wy_nl<T, PB, 1>(P, pld, Tm, rows, B, ldb, c0, c1, trans, red);
@@ -944,10 +972,14 @@ __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
     T *red = Tm + 2 * PB * PB;
     for (int c = threadIdx.x; c < ncols; c += NT) w.piv[c] = c;
     __syncthreads();
+    // with FP64 16-column panels the trailing update leaves the next panel's column norms in cn
+    // (the core buffer, unused until the core is built); otherwise each panel sums them itself
+    constexpr bool fused_norms = PB == 16 && sizeof(T) == 8;
+    T *cn = w.j;
     T stop2 = T(-1);
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
         const int jb = (nsteps - j0) < PB ? (nsteps - j0) : PB;
-        const T trail2 = window_pivot<T, PB>(m, ncols, w.y, m, j0, jb, w.piv, P);
+        const T trail2 = window_pivot<T, PB>(m, ncols, w.y, m, j0, jb, w.piv, P, (fused_norms && j0 > 0) ? cn : nullptr);
         if (j0 == 0) {
             stop2 = qr_y_stop2<T>(d, trail2, z.kk);
         } else if (trail2 <= stop2) {
@@ -960,7 +992,8 @@ __device__ void ph_qr_y(const RecompDesc<T> &d, T *lds) {
         }
         wg_geqrf_panel<T, PB>(m, ncols, w.y, m, w.tau_y, w.t_y, lds, nullptr, w.hdr, j0);
         if (j0 + jb < ncols)
-            apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, w.y + j0, m, j0 + jb, ncols, true, red);
+            apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, w.y + j0, m, j0 + jb, ncols, true, red,
+                                     fused_norms ? cn : nullptr, jb);
         __syncthreads();
     }
 }
