@@ -27,6 +27,24 @@ struct Eps<float> {
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
 
+// A thread's walk over the elements start, start + step, ... of a column-major array of `rows`
+// rows (rows > 0), as (row i, column c) pairs: the divisions are done once, at set-up, and each
+// step adds the step's row and column parts, carrying one column when the row wraps.  The walk
+// covers the first `cols` columns while c < cols.
+struct ColWalk {
+    int i, c, di, dc, rows;
+    __device__ __forceinline__ ColWalk(int start, int step, int rows_)
+        : i(start % rows_), c(start / rows_), di(step % rows_), dc(step / rows_), rows(rows_) {}
+    __device__ __forceinline__ void next() {
+        i += di;
+        c += dc;
+        if (i >= rows) {
+            i -= rows;
+            ++c;
+        }
+    }
+};
+
 
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {

@@ -450,7 +450,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
             __syncthreads();
             ph_core_size<T>(d, lds);
             ph_learned_size<T>(d, lds);
-            ph_core_build<T>(d);
+            ph_core_build<T, false>(d);
             ph_core_qr<T, PB>(d, lds);
             t_qry = HCAMD_TSTAMP();
             ph_svd_nov<T>(d, lds);

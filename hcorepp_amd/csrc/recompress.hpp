@@ -1081,21 +1081,21 @@ __device__ void ph_core_size(const RecompDesc<T> &d, T *lds) {
     __syncthreads();
 }
 
-template <typename T>
+// WITH_V: also set V1 = I for the Jacobi that accumulates the right rotations (the batched
+// form's Jacobi keeps no V and skips it).
+template <typename T, bool WITH_V = true>
 __device__ void ph_core_build(const RecompDesc<T> &d) {
     Ws<T> w = ws_of(d);
     const Dims z = dims_of(d, w);
     const int mu = z.mu, kt = z.kt, n = d.n;
     // J[piv[i], j] = R_Y[j, i]: rows of J in the order of Q_U's basis (undoing the column pivoting
     // of the QR of Y), so the left factor needs no permutation in the reconstruction
-    for (int64_t e = threadIdx.x; e < (int64_t) mu * kt; e += NT) {
-        const int64_t i = e % mu, j = e / mu;
-        w.j[w.piv[i] + j * mu] = (i >= j) ? w.y[j + i * n] : T(0);
-    }
-    for (int64_t e = threadIdx.x; e < (int64_t) kt * kt; e += NT) {
-        const int64_t i = e % kt, j = e / kt;
-        w.v1[e] = (i == j) ? T(1) : T(0);
-    }
+    if (mu > 0)
+        for (ColWalk p(threadIdx.x, NT, mu); p.c < kt; p.next())
+            w.j[w.piv[p.i] + (int64_t) p.c * mu] = (p.i >= p.c) ? w.y[p.c + (int64_t) p.i * n] : T(0);
+    if (WITH_V && kt > 0)
+        for (ColWalk p(threadIdx.x, NT, kt); p.c < kt; p.next())
+            w.v1[p.i + (int64_t) p.c * kt] = (p.i == p.c) ? T(1) : T(0);
     __syncthreads();
 }
 
@@ -1119,10 +1119,8 @@ __device__ void ph_core_qr(const RecompDesc<T> &d, T *lds) {
     if (kt < CORE_QR_MIN_COLS || 4 * mu < 5 * kt) return;
     wg_geqrf<T, PB>(mu, kt, w.j, mu, w.tau_j, w.t_j, lds);
     __syncthreads();
-    for (int64_t e = threadIdx.x; e < (int64_t) kt * kt; e += NT) {
-        const int64_t i = e % kt, c = e / kt;
-        w.j2[e] = (i <= c) ? w.j[i + c * mu] : T(0);
-    }
+    for (ColWalk p(threadIdx.x, NT, kt); p.c < kt; p.next())
+        w.j2[p.i + (int64_t) p.c * kt] = (p.i <= p.c) ? w.j[p.i + (int64_t) p.c * mu] : T(0);
     if (threadIdx.x == 0) w.hdr[HDR_CQR] = 1;
     __syncthreads();
 }
@@ -1394,8 +1392,8 @@ __device__ void ph_recon_nov(const RecompDesc<T> &d, T *lds) {
     const T *zc = cqr ? w.j2 : w.j;
     const int xrows = cqr ? kt : mu;
     T *U = d.cdata;  // ld m
-    for (int64_t e = threadIdx.x; e < (int64_t) m * r; e += NT) {
-        const int i = (int) (e % m), c = (int) (e / m);
+    for (ColWalk p(threadIdx.x, NT, m); p.c < r; p.next()) {
+        const int i = p.i, c = p.c;
         T v = T(0);
         if (i < xrows && c < kt) {
             const int j = w.perm[c];
@@ -1404,10 +1402,8 @@ __device__ void ph_recon_nov(const RecompDesc<T> &d, T *lds) {
         }
         U[i + (int64_t) c * m] = v;
     }
-    for (int64_t e = threadIdx.x; e < (int64_t) (n - kt) * r; e += NT) {
-        const int i = kt + (int) (e % (n - kt)), c = (int) (e / (n - kt));
-        w.vtmp[i + (int64_t) c * n] = T(0);
-    }
+    if (n > kt)
+        for (ColWalk p(threadIdx.x, NT, n - kt); p.c < r; p.next()) w.vtmp[kt + p.i + (int64_t) p.c * n] = T(0);
     __syncthreads();
     HCAMD_SUB_BEGIN(t_rf);
     const int tm = (kt + GT_M - 1) / GT_M, tn = (r + GT_N - 1) / GT_N;
@@ -1458,8 +1454,8 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
     if (side == 0 && w.hdr[HDR_CQR]) {
         T *B = d.cdata;  // ld m
         // left factor of the core J = Q2 R2: Q2 [Z2 / sigma; 0] in rows [0, mu), zero below
-        for (int64_t e = threadIdx.x; e < (int64_t) m * (c1 - c0); e += NT) {
-            const int i = (int) (e % m), c = c0 + (int) (e / m);
+        for (ColWalk p(threadIdx.x, NT, m); p.c < c1 - c0; p.next()) {
+            const int i = p.i, c = c0 + p.c;
             T v = T(0);
             if (i < kt && c < kt) {
                 const int j = w.perm[c];
@@ -1477,8 +1473,8 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
     } else if (side == 0) {
         T *B = d.cdata;  // ld m
         // the core's left factor Z / sigma in rows [0, mu) (J's rows are in Q_U's basis order)
-        for (int64_t e = threadIdx.x; e < (int64_t) m * (c1 - c0); e += NT) {
-            const int i = (int) (e % m), c = c0 + (int) (e / m);
+        for (ColWalk p(threadIdx.x, NT, m); p.c < c1 - c0; p.next()) {
+            const int i = p.i, c = c0 + p.c;
             T v = T(0);
             if (i < mu && c < kt) {
                 const int j = w.perm[c];
@@ -1493,8 +1489,8 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
         HCAMD_SUB_END(t_aq, w.hdr, 4);
     } else {
         T *B = w.vtmp;  // ld n
-        for (int64_t e = threadIdx.x; e < (int64_t) n * (c1 - c0); e += NT) {
-            const int i = (int) (e % n), c = c0 + (int) (e / n);
+        for (ColWalk p(threadIdx.x, NT, n); p.c < c1 - c0; p.next()) {
+            const int i = p.i, c = c0 + p.c;
             T v = T(0);
             if (i < kt && c < kt) {
                 const int j = w.perm[c];
@@ -1515,14 +1511,13 @@ __device__ void ph_recon(const RecompDesc<T> &d, int side, int c0, int c1, T *ld
 template <typename T>
 __device__ void ph_finish(const RecompDesc<T> &d, int wg, int nwg) {
     Ws<T> w = ws_of(d);
-    const int64_t r = w.hdr[HDR_RANK];
-    const int64_t n = d.n;
+    const int r = (int) w.hdr[HDR_RANK];
+    const int n = d.n;
     T *cv = d.cdata + (int64_t) d.m * d.max_rank;
-    const int64_t stride = (int64_t) nwg * NT;
-    for (int64_t e = (int64_t) wg * NT + threadIdx.x; e < r * n; e += stride) {
-        const int64_t c = e % r, i = e / r;
-        cv[e] = w.vtmp[i + c * n];
-    }
+    if (r < 1) return;
+    // element (c, i) of C_V is cv[c + i r]: the walk's rows are C_V's rows (the rank)
+    for (ColWalk p(wg * NT + threadIdx.x, nwg * NT, r); p.c < n; p.next())
+        cv[p.i + (int64_t) p.c * r] = w.vtmp[p.c + (int64_t) p.i * n];
 }
 
 }  // namespace hcamd
