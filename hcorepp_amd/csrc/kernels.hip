@@ -432,7 +432,9 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         const uint64_t t_start = HCAMD_TSTAMP();
         run_temp_gemms<T>(e, lds, mixed);
         const uint64_t t_gemm = HCAMD_TSTAMP();
-        ph_stack<T>(d, 0, 1);
+        // FP64 with 16-column panels: the QR of U_s reads U_s in place (StackedU), no stacked copy
+        constexpr bool u_in_place = PB == 16 && sizeof(T) == 8;
+        ph_stack<T, !u_in_place>(d, 0, 1);
         __syncthreads();
         const Dims z = dims_of(d, w);
         // features for the rank model, and in the exact form, whose runs are its training data;
@@ -440,7 +442,7 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         if (z.K <= d.kmax && (d.rank_model || !(d.model_tol > 0.0))) ph_features<T>(d, lds);
         uint64_t t_qru = 0, t_y = 0, t_qry = 0, t_svd = 0, t_ru = 0;
         if (z.K <= d.kmax) {
-            wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds, nullptr, w.hdr);
+            wg_geqrf<T, PB>(d.m, z.K, w.us, d.m, w.tau_u, w.t_u, lds, nullptr, w.hdr, u_in_place, stacked_u(d, z.K));
             __syncthreads();
             t_qru = HCAMD_TSTAMP();
             ph_y<T>(d, 0, 1, lds);

@@ -183,7 +183,9 @@ __device__ inline Dims dims_of(const RecompDesc<T> &d, const Ws<T> &w) {
 // Phase 1: stack.  Grid-stride so several workgroups may share one update
 // (`wg` in [0, nwg)).
 // ---------------------------------------------------------------------------
-template <typename T>
+// COPY false: the header words only (the batched FP64 form's QR of U_s reads U_s in place through
+// StackedU and writes it into the workspace itself).
+template <typename T, bool COPY = true>
 __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
     Ws<T> w = ws_of(d);
     const int64_t rc = *d.crank;
@@ -197,7 +199,7 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
         w.hdr[HDR_KQ] = 0;
         w.hdr[HDR_TRAIL2] = 0;
     }
-    if (K > d.kmax) return;
+    if (K > d.kmax || !COPY) return;
     (void) n;
     const T *cu = d.cdata;
     const T *af = d.afin;
@@ -242,6 +244,35 @@ template <typename T>
 __device__ inline StackedV<T> stacked_v(const RecompDesc<T> &d, int K) {
     const int rc = K - d.arank;
     return StackedV<T>{d.cdata + (int64_t) d.m * d.max_rank, d.bfin, d.beta, rc, d.ldbf};
+}
+
+// U_s[i, j] of the stacked left factor (m x K): C_U[i, j] for j < rc, alpha * Afin[i, j - rc]
+// after, as ph_stack copies it, read in place from C's U factor (m x rc, ld m) and Afin (ld
+// ldaf).  The QR of U_s reads it so in its first panel and first trailing update, which write
+// U_s's columns into the workspace as they go.
+template <typename T>
+struct StackedU {
+    const T *cu, *af;
+    T alpha;
+    int rc, ldu;
+    int64_t ldaf;
+    __device__ __forceinline__ T operator()(int i, int j) const {
+        return j < rc ? cu[i + (int64_t) j * ldu] : alpha * af[i + (int64_t) (j - rc) * ldaf];
+    }
+    // column j's first element, and the factor its elements take
+    __device__ __forceinline__ const T *column(int j, T &scale) const {
+        if (j < rc) {
+            scale = T(1);
+            return cu + (int64_t) j * ldu;
+        }
+        scale = alpha;
+        return af + (int64_t) (j - rc) * ldaf;
+    }
+};
+
+template <typename T>
+__device__ inline StackedU<T> stacked_u(const RecompDesc<T> &d, int K) {
+    return StackedU<T>{d.cdata, d.afin, d.alpha, K - d.arank, d.m, d.ldaf};
 }
 
 // Leading dimension of a Householder panel of `rows` rows staged in LDS: odd, so that the 16
@@ -361,11 +392,14 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_nl(const T *P, int pld, const T *Tm, i
 // consecutive elements of its column per group.  P: panel in LDS (rows x 16, ld pld, unit
 // lower); Tm: 16 x 16 upper (LDS, zero below the diagonal).  With `cn`, the squared norms of the
 // updated columns over rows [nrm_lo, rows) go to cn[column] as the update writes them (the next
-// panel's pivoting reads them instead of passing over the trailing matrix once more).
+// panel's pivoting reads them instead of passing over the trailing matrix once more).  SRC: B's
+// columns are read from the stacked left factor `src` (rows from 0: the first trailing update of
+// the QR of U_s, B = U_s) and the result is written to B.
 // ---------------------------------------------------------------------------
+template <bool SRC = false>
 __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const double *Tm, int rows, double *B,
                                                int64_t ldb, int c0, int c1, bool trans, double *cn = nullptr,
-                                               int nrm_lo = 0) {
+                                               int nrm_lo = 0, StackedU<double> src = StackedU<double>{}) {
     const int l = lane_id(), i16 = l & 15, k4 = l >> 4;
     const int nblk = (c1 - c0 + 15) / 16;
     // A operand of W' = op(T) W, step s: op(T)[i16][4 s + k4]
@@ -382,6 +416,13 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
         const bool cok = col < c1;
         // lanes past c1 read column c0 (in range) and discard it
         double *bcol = B + (int64_t) (cok ? col : c0) * ldb;
+        const double *rcol = bcol;
+        double sc = 1.0;
+        if constexpr (SRC) rcol = src.column(cok ? col : c0, sc);
+        auto bload = [&](int r) -> double {
+            if constexpr (SRC) return sc * rcol[r];
+            else return bcol[r];
+        };
         // W = P^T B_blk over 64-row chunks: the 16 loads of B and of P of a chunk are in flight
         // together, and four accumulators (one per 16-row group) keep four MFMA chains apart
         hcamd_doublex4 acc[4];
@@ -393,7 +434,7 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
-                for (int s = 0; s < 4; ++s) bv[g][s] = bcol[r0 + 16 * g + 4 * k4 + s];
+                for (int s = 0; s < 4; ++s) bv[g][s] = bload(r0 + 16 * g + 4 * k4 + s);
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
@@ -410,7 +451,7 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
             for (int s = 0; s < 4; ++s) {
                 const int r = ra + s;
                 const double a = r < rows ? P[r + i16 * pld] : 0.0;
-                const double b = (r < rows && cok) ? bcol[r] : 0.0;
+                const double b = (r < rows && cok) ? bload(r) : 0.0;
                 acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[0], 0, 0, 0);
             }
         }
@@ -428,7 +469,7 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) old[g][v] = bcol[g0 + 16 * g + 4 * k4 + v];
+                for (int v = 0; v < 4; ++v) old[g][v] = bload(g0 + 16 * g + 4 * k4 + v);
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
@@ -456,7 +497,7 @@ __device__ HCAMD_PHASE_ATTR void apply_wy_mfma(const double *P, int pld, const d
             const int pa = g0 + prow, rq = g0 + 4 * k4;
             double old[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) old[v] = (rq + v < rows) ? bcol[rq + v] : 0.0;
+            for (int v = 0; v < 4; ++v) old[v] = (rq + v < rows) ? bload(rq + v) : 0.0;
             hcamd_doublex4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
@@ -735,7 +776,8 @@ __host__ __device__ inline bool window_pivot_fits(int64_t m, int64_t ncols, int 
 // LDS P (rows x PB, rows = m - j0) and T in LDS Tm; returns jb.
 template <typename T, int PB>
 __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv,
-                              int64_t *hdr, int j0, const T *cn = nullptr) {
+                              int64_t *hdr, int j0, const T *cn = nullptr, bool from_src = false,
+                              StackedU<T> src = StackedU<T>{}) {
     (void) hdr;
     T *P = lds;
     T *Tm = P + (int64_t) (m | 1) * PB;
@@ -751,13 +793,24 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
     HCAMD_SUB_BEGIN(t_piv);
     if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P, cn);
     HCAMD_SUB_END(t_piv, hdr, 5);
-    // one row per thread and iteration, the PB loads of a row in flight together
-    for (int r = threadIdx.x; r < rows; r += NT) {
-        T v[PB];
+    // one row per thread and iteration, the PB loads of a row in flight together; from_src (the
+    // first panel of U_s, j0 = 0): read from the stacked factor in place
+    if (from_src) {
+        for (int r = threadIdx.x; r < rows; r += NT) {
+            T v[PB];
 #pragma unroll
-        for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
+            for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? src(r, c) : T(0);
 #pragma unroll
-        for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
+            for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
+        }
+    } else {
+        for (int r = threadIdx.x; r < rows; r += NT) {
+            T v[PB];
+#pragma unroll
+            for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
+#pragma unroll
+            for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
+        }
     }
     __syncthreads();
     // Householder steps with one pass over the rows and one barrier per column.  The pass of step
@@ -880,10 +933,12 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
     return jb;
 }
 
-// The whole blocked QR in one workgroup: panels and trailing updates in turn.
+// The whole blocked QR in one workgroup: panels and trailing updates in turn.  from_src (FP64
+// 16-column panels only, the QR of U_s in the batched form): A is the stacked left factor `src`,
+// read in place by the first panel and the first trailing update, which write it into A.
 template <typename T, int PB>
 __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr,
-                         int64_t *hdr = nullptr) {
+                         int64_t *hdr = nullptr, bool from_src = false, StackedU<T> src = StackedU<T>{}) {
     T *P = lds;
     T *Tm = P + (int64_t) (m | 1) * PB;
     T *red = Tm + 2 * PB * PB;
@@ -893,10 +948,19 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
         __syncthreads();
     }
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
-        const int jb = wg_geqrf_panel<T, PB>(m, ncols, A, lda, tau, tsave, lds, piv, hdr, j0);
+        const bool first_src = from_src && j0 == 0;
+        const int jb = wg_geqrf_panel<T, PB>(m, ncols, A, lda, tau, tsave, lds, piv, hdr, j0, nullptr, first_src, src);
         HCAMD_SUB_BEGIN(t_up);
-        if (j0 + jb < ncols)
-            apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, A + j0, lda, j0 + jb, ncols, true, red);
+        if (j0 + jb < ncols) {
+            if constexpr (PB == 16 && sizeof(T) == 8) {
+                if (first_src)
+                    apply_wy_mfma<true>(P, panel_ld(m), Tm, m, A, lda, jb, ncols, true, nullptr, 0, src);
+                else
+                    apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, A + j0, lda, j0 + jb, ncols, true, red);
+            } else {
+                apply_wy_dispatch<T, PB>(P, panel_ld(m - j0), Tm, m - j0, A + j0, lda, j0 + jb, ncols, true, red);
+            }
+        }
         __syncthreads();
         HCAMD_SUB_END(t_up, hdr, 2);
     }
