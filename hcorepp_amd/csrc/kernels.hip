@@ -432,7 +432,8 @@ __global__ void HCAMD_KERNEL k_batched(const BatchEntry<T> *entries, const int32
         const uint64_t t_start = HCAMD_TSTAMP();
         run_temp_gemms<T>(e, lds, mixed);
         const uint64_t t_gemm = HCAMD_TSTAMP();
-        // FP64 with 16-column panels: the QR of U_s reads U_s in place (StackedU), no stacked copy
+        // FP64 with 16-column panels: the QR of U_s reads U_s beyond its first panel in place
+        // (StackedU); only the first panel's 16 columns are stacked
         constexpr bool u_in_place = PB == 16 && sizeof(T) == 8;
         ph_stack<T, !u_in_place>(d, 0, 1);
         __syncthreads();

@@ -183,8 +183,9 @@ __device__ inline Dims dims_of(const RecompDesc<T> &d, const Ws<T> &w) {
 // Phase 1: stack.  Grid-stride so several workgroups may share one update
 // (`wg` in [0, nwg)).
 // ---------------------------------------------------------------------------
-// COPY false: the header words only (the batched FP64 form's QR of U_s reads U_s in place through
-// StackedU and writes it into the workspace itself).
+// COPY false: the header words and U_s's first 16 columns only, the first panel of the QR (the
+// batched FP64 form's QR of U_s reads the other columns in place through StackedU in its first
+// trailing update, which writes them into the workspace).
 template <typename T, bool COPY = true>
 __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
     Ws<T> w = ws_of(d);
@@ -199,7 +200,7 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
         w.hdr[HDR_KQ] = 0;
         w.hdr[HDR_TRAIL2] = 0;
     }
-    if (K > d.kmax || !COPY) return;
+    if (K > d.kmax) return;
     (void) n;
     const T *cu = d.cdata;
     const T *af = d.afin;
@@ -208,7 +209,7 @@ __device__ void ph_stack(const RecompDesc<T> &d, int wg, int nwg) {
     // U_s only: V_s = [beta C_V^T | Bfin] is read in place by its one consumer, Y = V_s R_U^T
     // (ph_y).  m * K < 2^31 (m <= 2048, K <= 768), so the element index is 32-bit; four elements
     // per thread and pass have their loads in flight together.
-    const int mm = (int) m, totU = (int) (m * K), irc = (int) rc;
+    const int mm = (int) m, totU = (int) (m * (COPY || K < 16 ? K : 16)), irc = (int) rc;
     constexpr int EPT = 4;
     for (int e0 = wg * NT * EPT + (int) threadIdx.x; e0 < totU; e0 += nwg * NT * EPT) {
         T v[EPT];
@@ -248,17 +249,14 @@ __device__ inline StackedV<T> stacked_v(const RecompDesc<T> &d, int K) {
 
 // U_s[i, j] of the stacked left factor (m x K): C_U[i, j] for j < rc, alpha * Afin[i, j - rc]
 // after, as ph_stack copies it, read in place from C's U factor (m x rc, ld m) and Afin (ld
-// ldaf).  The QR of U_s reads it so in its first panel and first trailing update, which write
-// U_s's columns into the workspace as they go.
+// ldaf).  The QR of U_s reads it so in its first trailing update, which writes U_s's columns
+// beyond the first panel into the workspace as it goes.
 template <typename T>
 struct StackedU {
     const T *cu, *af;
     T alpha;
     int rc, ldu;
     int64_t ldaf;
-    __device__ __forceinline__ T operator()(int i, int j) const {
-        return j < rc ? cu[i + (int64_t) j * ldu] : alpha * af[i + (int64_t) (j - rc) * ldaf];
-    }
     // column j's first element, and the factor its elements take
     __device__ __forceinline__ const T *column(int j, T &scale) const {
         if (j < rc) {
@@ -776,8 +774,7 @@ __host__ __device__ inline bool window_pivot_fits(int64_t m, int64_t ncols, int 
 // LDS P (rows x PB, rows = m - j0) and T in LDS Tm; returns jb.
 template <typename T, int PB>
 __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv,
-                              int64_t *hdr, int j0, const T *cn = nullptr, bool from_src = false,
-                              StackedU<T> src = StackedU<T>{}) {
+                              int64_t *hdr, int j0, const T *cn = nullptr) {
     (void) hdr;
     T *P = lds;
     T *Tm = P + (int64_t) (m | 1) * PB;
@@ -793,24 +790,13 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
     HCAMD_SUB_BEGIN(t_piv);
     if (piv) window_pivot<T, PB>(m, ncols, A, lda, j0, jb, piv, P, cn);
     HCAMD_SUB_END(t_piv, hdr, 5);
-    // one row per thread and iteration, the PB loads of a row in flight together; from_src (the
-    // first panel of U_s, j0 = 0): read from the stacked factor in place
-    if (from_src) {
-        for (int r = threadIdx.x; r < rows; r += NT) {
-            T v[PB];
+    // one row per thread and iteration, the PB loads of a row in flight together
+    for (int r = threadIdx.x; r < rows; r += NT) {
+        T v[PB];
 #pragma unroll
-            for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? src(r, c) : T(0);
+        for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
 #pragma unroll
-            for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
-        }
-    } else {
-        for (int r = threadIdx.x; r < rows; r += NT) {
-            T v[PB];
-#pragma unroll
-            for (int c = 0; c < PB; ++c) v[c] = (c < jb) ? A[(j0 + r) + (int64_t) (j0 + c) * lda] : T(0);
-#pragma unroll
-            for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
-        }
+        for (int c = 0; c < PB; ++c) P[r + c * pld] = v[c];
     }
     __syncthreads();
     // Householder steps with one pass over the rows and one barrier per column.  The pass of step
@@ -934,8 +920,9 @@ __device__ int wg_geqrf_panel(int m, int ncols, T *A, int64_t lda, T *tau, T *ts
 }
 
 // The whole blocked QR in one workgroup: panels and trailing updates in turn.  from_src (FP64
-// 16-column panels only, the QR of U_s in the batched form): A is the stacked left factor `src`,
-// read in place by the first panel and the first trailing update, which write it into A.
+// 16-column panels only, the QR of U_s in the batched form): A holds only the first 16 columns of
+// the stacked left factor `src`; the first trailing update reads the others in place and writes
+// them into A.
 template <typename T, int PB>
 __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, T *lds, int32_t *piv = nullptr,
                          int64_t *hdr = nullptr, bool from_src = false, StackedU<T> src = StackedU<T>{}) {
@@ -949,7 +936,7 @@ __device__ void wg_geqrf(int m, int ncols, T *A, int64_t lda, T *tau, T *tsave, 
     }
     for (int j0 = 0; j0 < nsteps; j0 += PB) {
         const bool first_src = from_src && j0 == 0;
-        const int jb = wg_geqrf_panel<T, PB>(m, ncols, A, lda, tau, tsave, lds, piv, hdr, j0, nullptr, first_src, src);
+        const int jb = wg_geqrf_panel<T, PB>(m, ncols, A, lda, tau, tsave, lds, piv, hdr, j0);
         HCAMD_SUB_BEGIN(t_up);
         if (j0 + jb < ncols) {
             if constexpr (PB == 16 && sizeof(T) == 8) {
